@@ -1,0 +1,156 @@
+/* loom_amd.h — C ABI of the MI355X (gfx950) execution backend for Loom
+ * kernel_blank IR.
+ *
+ * The reference exposes its backends as C++ class templates (no FFI):
+ *   simd::compact_function<K>  (src/vm/simd.hpp:4329-4335, 3837-4008)
+ *   simd::hybrid_function<K>   (src/vm/simd.hpp:4310-4317, 4255-4279)
+ *   simd::hybrid_stream<K>     (src/vm/simd.hpp:4319-4327, 4281-4291,
+ *                               facade 2797-2903)
+ *   mp::Shop                   (src/vm/mp.hpp:316-385)
+ * The C++ facade in include/loom_vm/vm/hip.hpp keeps those call shapes and
+ * reaches the GPU only through the entry points below.  Kernels are compiled
+ * ahead of time into the library (or a module linked with it) and are
+ * addressed by index; nothing of C++ crosses this boundary.
+ *
+ * All entry points return 0 on success and a nonzero status otherwise (a HIP
+ * error code, or LOOM_E_* below); loom_last_error() describes the last
+ * failure of the calling thread.
+ */
+#ifndef LOOM_AMD_H
+#define LOOM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOOM_MAX_IO 16
+#define LOOM_MAX_BLOCKS 64
+#define LOOM_NUM_COUNTERS 16
+
+enum loom_status
+{
+    LOOM_OK          = 0,
+    LOOM_E_ARG       = 1000, /* invalid argument */
+    LOOM_E_KERNEL    = 1001, /* unknown kernel index or name */
+    LOOM_E_NODEVICE  = 1002, /* no GPU or library not initialised */
+    LOOM_E_RUNTIME   = 1003, /* a kernel raised an error flag */
+    LOOM_E_CAPACITY  = 1004, /* a size exceeds a configured capacity */
+    LOOM_E_STATE     = 1005  /* call not valid in the object's state */
+};
+
+/* scalar type codes of kernel inputs/outputs */
+enum loom_type
+{
+    LOOM_U8 = 0, LOOM_U16, LOOM_U32, LOOM_U64,
+    LOOM_I8, LOOM_I16, LOOM_I32, LOOM_I64,
+    LOOM_F32, LOOM_F64
+};
+
+enum loom_mode
+{
+    LOOM_MODE_COMPACT = 0, /* predicated wave64, per-wave reconvergence */
+    LOOM_MODE_HYBRID  = 1  /* persistent, in-wave refill + label-queue regroup */
+};
+
+/* counters reported by a run (indices into loom_run_stats.counters) */
+enum loom_counter
+{
+    LOOM_C_ACTIVE_OPS = 0, /* Σ popcount(EXEC)·ops(B) */
+    LOOM_C_SLOT_OPS,       /* Σ 64·ops(B): active-lane % = ACTIVE/SLOT */
+    LOOM_C_DISPATCHES,     /* wave block dispatches */
+    LOOM_C_THREAD_STEPS,   /* thread-block-steps */
+    LOOM_C_SPILL_ROWS,     /* threads spilled to regroup queues */
+    LOOM_C_SPILL_BYTES,
+    LOOM_C_FILL_ROWS,      /* threads filled from regroup queues */
+    LOOM_C_FILL_BYTES,
+    LOOM_C_ADMITTED,       /* threads admitted from the input */
+    LOOM_C_REFILLS,        /* in-wave refill events */
+    LOOM_C_BATCHES,        /* batches claimed */
+    LOOM_C_SPILL_FAILED    /* lanes kept because a queue was full */
+};
+
+/* error flag bits of loom_run_stats.errors */
+#define LOOM_ERR_ASSERT_FP   1u /* ops::assert_fp saw NaN or Inf */
+#define LOOM_ERR_STEP_BUDGET 2u /* a wave exceeded max_steps */
+
+typedef struct loom_kernel_info
+{
+    char     name[64];
+    uint64_t hash;        /* structural hash of the lowered kernel */
+    int32_t  n_inputs, n_outputs;
+    int32_t  input_type[LOOM_MAX_IO], output_type[LOOM_MAX_IO];
+    int32_t  n_blocks, n_registers, index_bytes;
+    uint32_t state_size;      /* Σ register widths incl. PC (core.hpp:1027) */
+    uint32_t queue_row_bytes; /* Σ over labels of 4 + live-in bytes */
+    uint16_t block_ops[LOOM_MAX_BLOCKS];        /* #Compute + 1 */
+    uint32_t block_live_bytes[LOOM_MAX_BLOCKS]; /* 4 + live-in bytes */
+} loom_kernel_info;
+
+typedef struct loom_params
+{
+    uint32_t threshold;      /* hybrid regroup threshold in lanes (0..63) */
+    uint32_t refill;         /* hybrid: admit new threads into free lanes */
+    uint32_t waves_per_cu;   /* hybrid persistent grid; 0 = occupancy limit */
+    uint32_t collect_stats;  /* fill loom_run_stats.counters */
+    uint64_t queue_capacity; /* hybrid rows per label queue; 0 = automatic */
+    uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40 */
+} loom_params;
+
+typedef struct loom_run_stats
+{
+    double   ms_total;      /* whole run on the stream (HIP events) */
+    double   ms_main;       /* Σ duration of the main kernel launches */
+    uint32_t main_launches; /* launches of the main kernel */
+    uint32_t passes;        /* hybrid passes */
+    uint32_t errors;        /* LOOM_ERR_* bits */
+    uint32_t grid_waves;    /* waves of the (persistent) grid */
+    uint64_t counters[LOOM_NUM_COUNTERS];
+} loom_run_stats;
+
+/* ---- device and registry ------------------------------------------------ */
+/* Replaces nothing in the reference (CPU backends need no device); mirrors
+ * the CUDA test backend's implicit device selection (src/vm/cuda.hpp:34-116). */
+int         loom_init(int device);
+const char* loom_last_error(void);
+int         loom_device_info(int* compute_units, int* wave_size,
+                             size_t* global_mem, char* name, size_t name_len);
+/* Kernel lookup: the reference binds kernels at compile time through the
+ * template parameter K (simd.hpp:3742-3748); here by name or index. */
+int loom_kernel_count(void);
+int loom_kernel_info_get(int index, loom_kernel_info* out);
+int loom_kernel_find(const char* name);
+
+/* ---- device memory (the reference owns SoA columns in bve::Memory,
+ * src/vm/bve.hpp:494-693; the CUDA backend in DeviceAllocator,
+ * src/vm/cuda.hpp:86-116) --------------------------------------------------- */
+int loom_malloc(void** ptr, size_t bytes);
+int loom_free(void* ptr);
+int loom_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int loom_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int loom_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int loom_memset(void* dst, int value, size_t bytes);
+int loom_synchronize(void);
+
+/* ---- function execution --------------------------------------------------
+ * Replaces compact_function<K>::operator() (simd.hpp:3837-4008) and
+ * hybrid_function<K>::operator() (simd.hpp:4262-4278): runs every thread of
+ * the inputs to HALT.  `inputs`/`outputs` hold one device column per kernel
+ * input/output (structure of arrays, element type per loom_kernel_info).
+ * Outputs are written in input order (the reference hybrid returns them in
+ * its sorted order; input order is the stricter contract). */
+int loom_run(int kernel, int mode, const void* const* inputs,
+             void* const* outputs, uint64_t n, const loom_params* params,
+             loom_run_stats* stats);
+/* The same with host columns (copies in and out; device buffers cached). */
+int loom_run_host(int kernel, int mode, const void* const* inputs,
+                  void* const* outputs, uint64_t n, const loom_params* params,
+                  loom_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOM_AMD_H */

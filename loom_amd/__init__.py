@@ -1,0 +1,234 @@
+"""loom_amd — MI355X (gfx950) execution backend for Loom kernel_blank IR.
+
+Python mirror of the reference backend interface (the C++ facade for C++
+callers is include/loom_vm/vm/hip.hpp):
+
+    f = loom_amd.compact_function("fib_u64")    # simd::compact_function<K>
+    (c,) = f(n)                                  # one host column per input
+    h = loom_amd.hybrid_function("fib_u64")     # simd::hybrid_function<K>
+    (c,) = h(n)
+
+Inputs are numpy arrays (host) or DeviceArray (HBM-resident columns);
+outputs come back in input order.  Everything goes through the C ABI of
+include/loom_amd.h; nothing here computes a kernel on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+TYPE_DTYPES = {
+    0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64,
+    4: np.int8, 5: np.int16, 6: np.int32, 7: np.int64,
+    8: np.float32, 9: np.float64,
+}
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    """Select the GPU (reference: implicit in the CUDA backend)."""
+    global _initialised
+    if not _initialised:
+        abi.check(abi.load().loom_init(device), "loom_init")
+        _initialised = True
+
+
+def device_info() -> dict:
+    init()
+    cus, ws, mem = C.c_int(), C.c_int(), C.c_size_t()
+    name = C.create_string_buffer(64)
+    abi.check(abi.load().loom_device_info(C.byref(cus), C.byref(ws), C.byref(mem), name, 64),
+              "loom_device_info")
+    return {"compute_units": cus.value, "wave_size": ws.value,
+            "global_mem": mem.value, "arch": name.value.decode()}
+
+
+@dataclass
+class Kernel:
+    """Summary of a lowered kernel (loom_kernel_info)."""
+    index: int
+    name: str
+    hash: int
+    inputs: list
+    outputs: list
+    n_blocks: int
+    n_registers: int
+    index_bytes: int
+    state_size: int
+    queue_row_bytes: int
+    block_ops: list = field(default_factory=list)
+    block_live_bytes: list = field(default_factory=list)
+
+
+def kernel(name_or_index) -> Kernel:
+    lib = abi.load()
+    if isinstance(name_or_index, Kernel):
+        return name_or_index
+    if isinstance(name_or_index, str):
+        idx = lib.loom_kernel_find(name_or_index.encode())
+        if idx < 0:
+            raise KeyError(f"no kernel named {name_or_index!r}")
+    else:
+        idx = int(name_or_index)
+    info = abi.KernelInfo()
+    abi.check(lib.loom_kernel_info_get(idx, C.byref(info)), "loom_kernel_info_get")
+    nb = info.n_blocks
+    return Kernel(
+        index=idx, name=info.name.decode(), hash=info.hash,
+        inputs=[TYPE_DTYPES[info.input_type[k]] for k in range(info.n_inputs)],
+        outputs=[TYPE_DTYPES[info.output_type[k]] for k in range(info.n_outputs)],
+        n_blocks=nb, n_registers=info.n_registers, index_bytes=info.index_bytes,
+        state_size=info.state_size, queue_row_bytes=info.queue_row_bytes,
+        block_ops=list(info.block_ops[:nb]), block_live_bytes=list(info.block_live_bytes[:nb]))
+
+
+def kernels() -> list[Kernel]:
+    lib = abi.load()
+    return [kernel(i) for i in range(lib.loom_kernel_count())]
+
+
+class DeviceArray:
+    """A 1-D column in device memory (owned)."""
+
+    def __init__(self, n: int, dtype):
+        init()
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        abi.check(abi.load().loom_malloc(C.byref(p), max(1, self.nbytes)), "loom_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            abi.check(abi.load().loom_memcpy_h2d(d.ptr, a.ctypes.data, d.nbytes), "h2d")
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            abi.check(abi.load().loom_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def zero(self) -> None:
+        abi.check(abi.load().loom_memset(self.ptr, 0, max(1, self.nbytes)), "memset")
+
+    def free(self) -> None:
+        if self.ptr:
+            abi.load().loom_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class RunResult:
+    outputs: list
+    ms_total: float
+    ms_main: float
+    main_launches: int
+    passes: int
+    errors: int
+    grid_waves: int
+    counters: list
+
+    @property
+    def active_lane_fraction(self) -> float:
+        slot = self.counters[abi.C_SLOT_OPS]
+        return self.counters[abi.C_ACTIVE_OPS] / slot if slot else float("nan")
+
+
+def make_params(threshold=0, refill=True, waves_per_cu=0, queue_capacity=0, max_steps=0,
+                stats=True) -> abi.Params:
+    p = abi.Params()
+    p.threshold = int(threshold)
+    p.refill = 1 if refill else 0
+    p.waves_per_cu = int(waves_per_cu)
+    p.queue_capacity = int(queue_capacity)
+    p.max_steps = int(max_steps)
+    p.collect_stats = 1 if stats else 0
+    return p
+
+
+def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = None) -> RunResult:
+    """Run every thread of `inputs` to HALT on the GPU.
+
+    `inputs` are numpy arrays (copied in/out through the C ABI) or
+    DeviceArray columns (then `outputs` must be DeviceArray columns too and
+    the run reads and writes HBM only)."""
+    init()
+    k = kernel(k)
+    lib = abi.load()
+    if len(inputs) != len(k.inputs):
+        raise ValueError(f"{k.name} takes {len(k.inputs)} inputs, got {len(inputs)}")
+    params = params or make_params()
+    st = abi.RunStats()
+    device = isinstance(inputs[0], DeviceArray)
+    if n is None:
+        n = inputs[0].n if device else len(inputs[0])
+    if device:
+        if outputs is None:
+            outputs = [DeviceArray(n, dt) for dt in k.outputs]
+        for a, dt in zip(inputs, k.inputs):
+            if a.dtype != np.dtype(dt) or a.n < n:
+                raise ValueError("device input column of wrong type or size")
+        ins = (C.c_void_p * abi.MAX_IO)(*[a.ptr for a in inputs])
+        outs = (C.c_void_p * abi.MAX_IO)(*[a.ptr for a in outputs])
+        abi.check(lib.loom_run(k.index, mode, ins, outs, n, C.byref(params), C.byref(st)),
+                  "loom_run")
+    else:
+        host_in = [np.ascontiguousarray(a, dtype=dt) for a, dt in zip(inputs, k.inputs)]
+        for a in host_in:
+            if len(a) != n:
+                raise ValueError("input columns differ in length")
+        if outputs is None:
+            outputs = [np.zeros(n, dt) for dt in k.outputs]
+        ins = (C.c_void_p * abi.MAX_IO)(*[a.ctypes.data for a in host_in])
+        outs = (C.c_void_p * abi.MAX_IO)(*[a.ctypes.data for a in outputs])
+        abi.check(lib.loom_run_host(k.index, mode, ins, outs, n, C.byref(params),
+                                    C.byref(st)), "loom_run_host")
+    return RunResult(outputs=outputs, ms_total=st.ms_total, ms_main=st.ms_main,
+                     main_launches=st.main_launches, passes=st.passes, errors=st.errors,
+                     grid_waves=st.grid_waves, counters=list(st.counters))
+
+
+class compact_function:
+    """Predicated wave64 form (reference simd::compact_function<K>,
+    src/vm/simd.hpp:4329-4335): `f(*inputs)` returns the output columns."""
+
+    mode = abi.MODE_COMPACT
+
+    def __init__(self, k, **params):
+        self.kernel = kernel(k)
+        self.params = make_params(**params)
+        self.last: RunResult | None = None
+
+    def __call__(self, *inputs):
+        self.last = run(self.kernel, self.mode, list(inputs), params=self.params)
+        return tuple(self.last.outputs)
+
+
+class hybrid_function(compact_function):
+    """Regrouping persistent form (reference simd::hybrid_function<K>,
+    src/vm/simd.hpp:4310-4317).  `threshold` is the regroup threshold in lanes
+    (the reference UTRt·N, simd.hpp:4094-4098), `refill` admits new threads
+    into free lanes, `waves_per_cu` sizes the persistent grid."""
+
+    mode = abi.MODE_HYBRID
+
+    def __init__(self, k, threshold=32, refill=True, waves_per_cu=0, queue_capacity=0,
+                 **params):
+        super().__init__(k, threshold=threshold, refill=refill, waves_per_cu=waves_per_cu,
+                         queue_capacity=queue_capacity, **params)

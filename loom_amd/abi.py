@@ -1,0 +1,126 @@
+"""ctypes binding of the C ABI declared in include/loom_amd.h.
+
+This is the Python side of the drop-in boundary: plain pointers and sizes,
+no torch types.  Loading fails loudly when the library was not built; there
+is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "_build" / "libloom_amd.so"
+
+MAX_IO = 16
+MAX_BLOCKS = 64
+NUM_COUNTERS = 16
+
+MODE_COMPACT = 0
+MODE_HYBRID = 1
+
+# loom_counter
+C_ACTIVE_OPS, C_SLOT_OPS, C_DISPATCHES, C_THREAD_STEPS = 0, 1, 2, 3
+C_SPILL_ROWS, C_SPILL_BYTES, C_FILL_ROWS, C_FILL_BYTES = 4, 5, 6, 7
+C_ADMITTED, C_REFILLS, C_BATCHES, C_SPILL_FAILED = 8, 9, 10, 11
+
+ERR_ASSERT_FP = 1
+ERR_STEP_BUDGET = 2
+
+
+class KernelInfo(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 64),
+        ("hash", C.c_uint64),
+        ("n_inputs", C.c_int32),
+        ("n_outputs", C.c_int32),
+        ("input_type", C.c_int32 * MAX_IO),
+        ("output_type", C.c_int32 * MAX_IO),
+        ("n_blocks", C.c_int32),
+        ("n_registers", C.c_int32),
+        ("index_bytes", C.c_int32),
+        ("state_size", C.c_uint32),
+        ("queue_row_bytes", C.c_uint32),
+        ("block_ops", C.c_uint16 * MAX_BLOCKS),
+        ("block_live_bytes", C.c_uint32 * MAX_BLOCKS),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("threshold", C.c_uint32),
+        ("refill", C.c_uint32),
+        ("waves_per_cu", C.c_uint32),
+        ("collect_stats", C.c_uint32),
+        ("queue_capacity", C.c_uint64),
+        ("max_steps", C.c_uint64),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double),
+        ("ms_main", C.c_double),
+        ("main_launches", C.c_uint32),
+        ("passes", C.c_uint32),
+        ("errors", C.c_uint32),
+        ("grid_waves", C.c_uint32),
+        ("counters", C.c_uint64 * NUM_COUNTERS),
+    ]
+
+
+# every symbol declared in include/loom_amd.h, with its signature
+SIGNATURES = {
+    "loom_init": (C.c_int, [C.c_int]),
+    "loom_last_error": (C.c_char_p, []),
+    "loom_device_info": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
+    "loom_kernel_count": (C.c_int, []),
+    "loom_kernel_info_get": (C.c_int, [C.c_int, C.POINTER(KernelInfo)]),
+    "loom_kernel_find": (C.c_int, [C.c_char_p]),
+    "loom_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "loom_free": (C.c_int, [C.c_void_p]),
+    "loom_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "loom_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "loom_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "loom_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
+    "loom_synchronize": (C.c_int, []),
+    "loom_run": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                           C.c_uint64, C.POINTER(Params), C.POINTER(RunStats)]),
+    "loom_run_host": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
+                                C.POINTER(RunStats)]),
+}
+
+_lib = None
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Load the kernel library (no GPU call is made here)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(
+                f"{_LIB_PATH} is missing: build it with `python -m loom_amd.build`")
+        lib = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class LoomError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = load().loom_last_error()
+        super().__init__(f"{where} failed ({code}): {msg.decode() if msg else ''}")
+        self.code = code
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise LoomError(code, where)

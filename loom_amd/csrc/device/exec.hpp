@@ -1,0 +1,320 @@
+// -*- C++ -*-
+/// @file exec.hpp
+/// @brief Lowering of basic blocks to gfx950 code: a per-lane register file
+/// and one inlined instruction sequence per block.
+///
+/// The register file is an array of 32-bit slots (registers of at most four
+/// bytes) and an array of 64-bit slots, indexed only by compile-time
+/// constants, so the compiler keeps every virtual register in VGPRs.  Each
+/// instruction is expanded from the constant kernel table into straight-line
+/// code; there is no interpretation at run time.  A block executes every one
+/// of its instructions; the program counter written last decides the next
+/// block (reference semantics, core.hpp:736-802, 804-829).
+#pragma once
+
+#include "ops.hpp"
+
+#include <bit>
+#include <cstdint>
+#include <utility>
+
+namespace loom::hipvm
+{
+using ir::halt16;
+using ir::SK;
+
+/// @brief Lane program counter value of a halted thread.
+inline constexpr std::uint32_t HALT = halt16;
+/// @brief "No block" result of the scheduler helpers.
+inline constexpr std::uint32_t NONE = 0xFFFFFFFEu;
+
+/// @brief Slot layout of kernel @p K's registers on a lane.
+template <typename K>
+struct Layout
+{
+    static constexpr const auto& T = ir::table_of<K>;
+
+    struct Map
+    {
+        std::array<std::uint16_t, ir::max_regs> idx{};
+        std::size_t                             n32 = 0, n64 = 0;
+    };
+
+    static constexpr Map map = []
+    {
+        Map m;
+        for(std::size_t r = 0; r < ir::table_of<K>.nr; ++r)
+            m.idx[r] = static_cast<std::uint16_t>(ir::table_of<K>.reg[r].width == 8 ? m.n64++ : m.n32++);
+        return m;
+    }();
+
+    static constexpr std::size_t n32 = map.n32 ? map.n32 : 1;
+    static constexpr std::size_t n64 = map.n64 ? map.n64 : 1;
+};
+
+template <typename K>
+struct Regs
+{
+    using L = Layout<K>;
+    std::uint32_t r32[L::n32];
+    std::uint64_t r64[L::n64];
+
+    template <std::size_t slot, typename V>
+    LOOM_DEVICE V
+    get() const
+    {
+        constexpr std::size_t i = L::map.idx[slot];
+        if constexpr(L::T.reg[slot].width == 8)
+            return std::bit_cast<V>(r64[i]);
+        else if constexpr(sizeof(V) == 4)
+            return std::bit_cast<V>(r32[i]);
+        else if constexpr(sizeof(V) == 2)
+            return std::bit_cast<V>(static_cast<std::uint16_t>(r32[i]));
+        else
+            return std::bit_cast<V>(static_cast<std::uint8_t>(r32[i]));
+    }
+
+    template <std::size_t slot, typename V>
+    LOOM_DEVICE void
+    set(V v)
+    {
+        constexpr std::size_t i = L::map.idx[slot];
+        if constexpr(L::T.reg[slot].width == 8)
+            r64[i] = std::bit_cast<std::uint64_t>(v);
+        else if constexpr(sizeof(V) == 4)
+            r32[i] = std::bit_cast<std::uint32_t>(v);
+        else if constexpr(sizeof(V) == 2)
+            r32[i] = std::bit_cast<std::uint16_t>(v);
+        else
+            r32[i] = std::bit_cast<std::uint8_t>(v);
+    }
+
+    /// @brief Raw slot bits (for spill/fill).
+    template <std::size_t slot>
+    LOOM_DEVICE std::uint64_t
+    raw() const
+    {
+        constexpr std::size_t i = L::map.idx[slot];
+        if constexpr(L::T.reg[slot].width == 8)
+            return r64[i];
+        else
+            return r32[i];
+    }
+
+    template <std::size_t slot>
+    LOOM_DEVICE void
+    set_raw(std::uint64_t v)
+    {
+        constexpr std::size_t i = L::map.idx[slot];
+        if constexpr(L::T.reg[slot].width == 8)
+            r64[i] = v;
+        else
+            r32[i] = static_cast<std::uint32_t>(v);
+    }
+};
+
+/// @brief Value of operand @p O (a literal or a register read).
+template <typename K, std::size_t O>
+LOOM_DEVICE auto
+operand(const Regs<K>& r)
+{
+    constexpr ir::Operand o = ir::table_of<K>.opnd[O];
+    using V                 = ctype<o.ty>;
+    if constexpr(o.lit)
+    {
+        if constexpr(sizeof(V) == 8)
+            return std::bit_cast<V>(static_cast<std::uint64_t>(o.bits));
+        else if constexpr(sizeof(V) == 4)
+            return std::bit_cast<V>(static_cast<std::uint32_t>(o.bits));
+        else if constexpr(sizeof(V) == 2)
+            return std::bit_cast<V>(static_cast<std::uint16_t>(o.bits));
+        else
+            return std::bit_cast<V>(static_cast<std::uint8_t>(o.bits));
+    }
+    else
+        return r.template get<o.slot, V>();
+}
+
+/// @brief Bits of a table constant as type @p V.
+template <typename V>
+LOOM_DEVICE V
+const_as(std::uint64_t bits)
+{
+    if constexpr(sizeof(V) == 8)
+        return std::bit_cast<V>(bits);
+    else if constexpr(sizeof(V) == 4)
+        return std::bit_cast<V>(static_cast<std::uint32_t>(bits));
+    else if constexpr(sizeof(V) == 2)
+        return std::bit_cast<V>(static_cast<std::uint16_t>(bits));
+    else
+        return std::bit_cast<V>(static_cast<std::uint8_t>(bits));
+}
+
+/// @brief Run-time error flags raised by kernels.
+enum ErrorFlag : std::uint32_t
+{
+    ERR_ASSERT_FP   = 1u << 0, ///< ops::assert_fp saw NaN/Inf
+    ERR_STEP_BUDGET = 1u << 1, ///< a wave exceeded its block-step budget
+    ERR_QUEUE_FULL  = 1u << 2, ///< a regroup queue overflowed (never expected)
+};
+
+/// @brief Execute instruction @p I of kernel @p K on this lane.
+template <typename K, std::size_t I>
+LOOM_DEVICE void
+exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
+{
+    constexpr ir::Instr   in = ir::table_of<K>.instr[I];
+
+    if constexpr(in.kind == SK::jump)
+        pc = ir::table_of<K>.target[in.t0];
+    else if constexpr(in.kind == SK::pjump)
+    {
+        const auto v = operand<K, in.a0>(r);
+        if constexpr(in.nt == 2)
+            pc = v != decltype(v)(0) ? std::uint32_t(ir::table_of<K>.target[in.t0 + 1]) :
+                                       std::uint32_t(ir::table_of<K>.target[in.t0]);
+        else
+        {
+            std::uint32_t p = HALT;
+            [&]<std::size_t... J>(std::index_sequence<J...>)
+            {
+                ((p = (v == decltype(v)(J)) ? std::uint32_t(ir::table_of<K>.target[in.t0 + J]) : p), ...);
+            }(std::make_index_sequence<in.nt>{});
+            pc = p;
+        }
+    }
+    else
+    {
+        constexpr ir::Operand d = ir::table_of<K>.opnd[in.dst];
+        using D                 = ctype<d.ty>;
+        using E                 = ctype<in.ety>;
+        constexpr Op op         = in.op;
+
+        if constexpr(op == Op::mov)
+            r.template set<d.slot, D>(conv<D>(operand<K, in.a0>(r)));
+        else if constexpr(op == Op::assert_fp)
+        {
+            const E v = conv<E>(operand<K, in.a0>(r));
+            if(!(v - v == E(0))) err |= ERR_ASSERT_FP;
+            r.template set<d.slot, D>(conv<D>(v));
+        }
+        else if constexpr(op == Op::cmov)
+        {
+            const auto c = operand<K, in.a0>(r);
+            const E    a = conv<E>(operand<K, in.a0 + 1>(r));
+            const E    b = conv<E>(operand<K, in.a0 + 2>(r));
+            r.template set<d.slot, D>(conv<D>(c != decltype(c)(0) ? a : b));
+        }
+        else if constexpr(op == Op::fma || op == Op::fms)
+        {
+            static_assert(std::is_floating_point_v<E>, "fma needs floating point");
+            const E x = conv<E>(operand<K, in.a0>(r));
+            const E y = conv<E>(operand<K, in.a0 + 1>(r));
+            const E z = conv<E>(operand<K, in.a0 + 2>(r));
+            r.template set<d.slot, D>(conv<D>(fma_<E>(x, y, op == Op::fma ? z : E(-z))));
+        }
+        else if constexpr(op == Op::sct)
+        {
+            using IX = ctype<in.ity>;
+            const IX i = conv<IX>(operand<K, in.a0>(r));
+            E       v = E(0);
+            [&]<std::size_t... J>(std::index_sequence<J...>)
+            {
+                ((v = (i == IX(J)) ? const_as<E>(ir::table_of<K>.konst[in.c0 + J]) : v), ...);
+            }(std::make_index_sequence<in.nc>{});
+            r.template set<d.slot, D>(conv<D>(v));
+        }
+        else if constexpr(op == Op::slt)
+        {
+            using IX = ctype<in.ity>;
+            const IX i = conv<IX>(operand<K, in.a0>(r));
+            E       v = E(0);
+            [&]<std::size_t... J>(std::index_sequence<J...>)
+            {
+                ((v = (i == IX(J)) ? conv<E>(operand<K, in.a0 + 1 + J>(r)) : v), ...);
+            }(std::make_index_sequence<in.na - 1>{});
+            r.template set<d.slot, D>(conv<D>(v));
+        }
+        else if constexpr(op == Op::pacc)
+        {
+            using P      = ctype<in.ity>;
+            const P p    = conv<P>(operand<K, in.a0>(r));
+            const E x    = conv<E>(operand<K, in.a0 + 1>(r));
+            const E y    = conv<E>(operand<K, in.a0 + 2>(r));
+            const bool c = arith<in.cmp, E>(x, y);
+            const bool v = in.acc == Op::land ? (p != P(0)) && c : (p != P(0)) || c;
+            r.template set<d.slot, D>(conv<D>(v));
+        }
+        else if constexpr(op == Op::neg || op == Op::lnot || op == Op::bnot ||
+                          op == Op::abs || op == Op::rsqrt || op == Op::round ||
+                          op == Op::ceil || op == Op::floor || op == Op::trunc ||
+                          op == Op::sin || op == Op::cos || op == Op::sqrt)
+        {
+            const E a = conv<E>(operand<K, in.a0>(r));
+            r.template set<d.slot, D>(conv<D>(arith<op, E>(a)));
+        }
+        else
+        {
+            const E a = conv<E>(operand<K, in.a0>(r));
+            const E b = conv<E>(operand<K, in.a0 + 1>(r));
+            r.template set<d.slot, D>(conv<D>(arith<op, E>(a, b)));
+        }
+    }
+}
+
+/// @brief Execute all instructions of block @p B.
+template <typename K, std::size_t B>
+LOOM_DEVICE void
+exec_block(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
+{
+    constexpr ir::Block blk = ir::table_of<K>.block[B];
+    [&]<std::size_t... J>(std::index_sequence<J...>)
+    { (exec_instr<K, blk.i0 + J>(r, pc, err), ...); }(std::make_index_sequence<blk.ni>{});
+    if constexpr(ir::table_of<K>.straightline) pc = HALT;
+}
+
+/// @brief Load kernel inputs of thread @p tid into the register file and
+/// zero every other register live at START.
+template <typename K>
+LOOM_DEVICE void
+load_inputs(Regs<K>& r, const void* const* in, std::uint64_t tid)
+{
+    [&]<std::size_t... R>(std::index_sequence<R...>)
+    {
+        (
+            [&]
+            {
+                if constexpr(ir::table_of<K>.block[0].live_in.test(R) && !ir::table_of<K>.reg[R].input)
+                    r.template set_raw<R>(0);
+            }(),
+            ...);
+    }(std::make_index_sequence<ir::table_of<K>.nr>{});
+    [&]<std::size_t... J>(std::index_sequence<J...>)
+    {
+        (
+            [&]
+            {
+                using V = ctype<ir::table_of<K>.in_ty[J]>;
+                r.template set<ir::table_of<K>.in_slot[J], V>(static_cast<const V*>(in[J])[tid]);
+            }(),
+            ...);
+    }(std::make_index_sequence<ir::table_of<K>.nin>{});
+}
+
+/// @brief Store kernel outputs of thread @p tid.
+template <typename K>
+LOOM_DEVICE void
+store_outputs(const Regs<K>& r, void* const* out, std::uint64_t tid)
+{
+    [&]<std::size_t... J>(std::index_sequence<J...>)
+    {
+        (
+            [&]
+            {
+                using V = ctype<ir::table_of<K>.out_ty[J]>;
+                static_cast<V*>(out[J])[tid] = r.template get<ir::table_of<K>.out_slot[J], V>();
+            }(),
+            ...);
+    }(std::make_index_sequence<ir::table_of<K>.nout>{});
+}
+} // namespace loom::hipvm

@@ -1,0 +1,711 @@
+// -*- C++ -*-
+/// @file sched.hpp
+/// @brief Wave64 schedulers: the predicated "compact" form and the
+/// regrouping "hybrid" form.
+///
+/// Both keep a wave's 64 threads in VGPRs and choose, per step, one block
+/// for the whole wave; EXEC is the set of lanes whose program counter equals
+/// that block.  The choice follows the reference SUCCESSOR_MAJOR threaded
+/// dispatch (simd.hpp:3545-3732, 3974-4005): after a block runs, the most
+/// populated successor runs next while its population exceeds the threshold;
+/// otherwise the most populated block of the pending set is chosen, else of
+/// all blocks present in the wave (greedy groups, simd.hpp:2477-2531).
+/// Populations are `popcount(ballot(pc == b))`, ties go to the lower block
+/// index or the earlier successor, exactly as `major_group` (2606-2637).
+///
+/// The hybrid form (reference hybrid_compiler, simd.hpp:4016-4292) runs the
+/// same executor inside a persistent pass kernel.  A wave takes a batch of 64
+/// threads from one per-label queue (or 64 new threads), executes until the
+/// best block population drops to the threshold, refills free lanes with new
+/// threads (the stream exchange of simd.hpp:2878-2890 done in registers),
+/// and finally spills survivors into the queue of their current label: lanes
+/// of one label get consecutive rows from one atomic add (ballot + mbcnt
+/// rank), and only registers live at that label are written, column by
+/// column (structure of arrays).  The next pass drains those queues in full
+/// 64-row batches, which is the regroup (the counting sort of
+/// simd.hpp:2667-2745 without moving dead registers).
+#pragma once
+
+#include "exec.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace loom::hipvm
+{
+inline constexpr int         kMaxIO    = 16;
+inline constexpr std::size_t kNumStats = 12;
+
+/// @brief Kernel input/output columns (device pointers, structure of arrays).
+struct IoArgs
+{
+    const void* in[kMaxIO];
+    void*       out[kMaxIO];
+};
+
+/// @brief Counters accumulated per run (device, u64 each).
+enum Stat : int
+{
+    ST_ACTIVE_OPS = 0, ///< Σ popcount(EXEC)·ops(B) over block dispatches
+    ST_SLOT_OPS,       ///< Σ 64·ops(B)
+    ST_DISPATCHES,     ///< wave block dispatches
+    ST_THREAD_STEPS,   ///< Σ popcount(EXEC): thread-block-steps
+    ST_SPILL_ROWS,     ///< threads written to regroup queues
+    ST_SPILL_BYTES,    ///< bytes written by regroup spills
+    ST_FILL_ROWS,      ///< threads read from regroup queues
+    ST_FILL_BYTES,     ///< bytes read by regroup fills
+    ST_ADMITTED,       ///< threads admitted from the input
+    ST_REFILLS,        ///< in-wave refill events
+    ST_BATCHES,        ///< batches claimed by persistent waves
+    ST_SPILL_FAILED,   ///< lanes kept because a queue was full
+};
+
+/// @brief Per-run control arguments.
+struct CtlArgs
+{
+    std::uint64_t  n         = 0;       ///< number of threads
+    std::uint64_t  max_steps = 1ull << 40; ///< per-wave block dispatch budget
+    std::uint32_t  thr       = 0;       ///< regroup threshold (lanes)
+    std::uint32_t  refill    = 1;       ///< hybrid: refill free lanes in-wave
+    std::uint32_t* err       = nullptr; ///< error flags
+    std::uint64_t* stats     = nullptr; ///< kNumStats counters or null
+};
+
+/// @brief Hybrid control block, resident in device memory.
+struct HybridCtl
+{
+    std::uint64_t next_input;     ///< next thread index to admit
+    std::uint32_t batch_next;     ///< batch claim counter of the pass
+    std::uint32_t nbatches;       ///< queue batches of the pass
+    std::uint32_t parity;         ///< in-queue side of the pass
+    std::uint32_t live;           ///< threads queued for the pass
+    std::uint32_t passes;         ///< passes run so far
+    std::uint32_t done;           ///< set when no thread remains
+    std::uint32_t qcount[2][64];  ///< per side, per label queue fill
+    std::uint32_t bprefix[65];    ///< batch prefix over labels of the pass
+};
+
+struct HybridArgs
+{
+    HybridCtl* ctl = nullptr;
+    char*      q[2]{nullptr, nullptr}; ///< queue buffers (two sides)
+    std::uint64_t qcap = 0;            ///< rows per label queue (multiple of 64)
+};
+
+// ---------------------------------------------------------------------------
+// wave primitives
+// ---------------------------------------------------------------------------
+LOOM_DEVICE std::uint64_t
+ballot(bool p)
+{
+    return __ballot(p);
+}
+
+LOOM_DEVICE std::uint32_t
+popc(std::uint64_t m)
+{
+    return static_cast<std::uint32_t>(__popcll(m));
+}
+
+LOOM_DEVICE std::uint32_t
+lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+/// @brief Number of set bits of @p m below this lane.
+LOOM_DEVICE std::uint32_t
+rank_in(std::uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<std::uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<std::uint32_t>(m), 0u));
+}
+
+LOOM_DEVICE std::uint32_t
+uniform(std::uint32_t v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+LOOM_DEVICE std::uint64_t
+uniform64(std::uint64_t v)
+{
+    const std::uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<std::uint32_t>(v));
+    const std::uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<std::uint32_t>(v >> 32));
+    return (std::uint64_t(hi) << 32) | lo;
+}
+
+/// @brief Per-wave scheduling state (wave-uniform values).
+struct WaveCtx
+{
+    std::uint64_t act = 0, slots = 0, disp = 0, tsteps = 0;
+    std::uint64_t max_steps = 0;
+    std::uint32_t err = 0;   ///< lane-local error bits
+    bool          over = false;
+};
+
+/// @brief Run block @p B on the lanes at B and pick the next block among its
+/// successors (threaded dispatch, simd.hpp:3577-3731).
+template <typename K, std::size_t B>
+LOOM_DEVICE std::uint32_t
+step(Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, WaveCtx& w)
+{
+    constexpr ir::Block   blk = ir::table_of<K>.block[B];
+
+    const std::uint64_t m = ballot(pc == B);
+    if(pc == B) exec_block<K, B>(r, pc, w.err);
+    const std::uint32_t n = popc(m);
+    w.act += std::uint64_t(n) * blk.ops;
+    w.slots += 64ull * blk.ops;
+    w.disp += 1;
+    w.tsteps += n;
+    pending &= ~(1ull << B);
+
+    if constexpr(blk.ns == 0)
+        return NONE;
+    else if constexpr(blk.ns == 1)
+    {
+        constexpr std::uint32_t s = ir::table_of<K>.succ[blk.s0];
+        if constexpr(s == HALT)
+            return NONE;
+        else
+            return popc(ballot(pc == s)) > thr ? s : NONE;
+    }
+    else
+    {
+        std::uint32_t best = NONE, bp = 0;
+        [&]<std::size_t... J>(std::index_sequence<J...>)
+        {
+            (
+                [&]
+                {
+                    constexpr std::uint32_t s = ir::table_of<K>.succ[blk.s0 + J];
+                    const std::uint32_t     p = popc(ballot(pc == s));
+                    if constexpr(s != HALT)
+                        if(p) pending |= 1ull << s;
+                    if(p > bp)
+                    {
+                        bp   = p;
+                        best = s;
+                    }
+                }(),
+                ...);
+        }(std::make_index_sequence<blk.ns>{});
+        return (bp > thr && best != HALT) ? best : NONE;
+    }
+}
+
+#define LOOM_CASE(b)                                                        \
+    case b:                                                                 \
+        if constexpr(b < ir::table_of<K>.nb)                                \
+            return step<K, (b < ir::table_of<K>.nb ? b : 0)>(r, pc, pending, thr, w); \
+        else                                                                \
+            return NONE;
+
+/// @brief Jump to the step of block @p cur (a wave-uniform value).
+template <typename K>
+LOOM_DEVICE std::uint32_t
+dispatch(std::uint32_t cur, Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending,
+         std::uint32_t thr, WaveCtx& w)
+{
+    switch(cur)
+    {
+        LOOM_CASE(0) LOOM_CASE(1) LOOM_CASE(2) LOOM_CASE(3) LOOM_CASE(4)
+        LOOM_CASE(5) LOOM_CASE(6) LOOM_CASE(7) LOOM_CASE(8) LOOM_CASE(9)
+        LOOM_CASE(10) LOOM_CASE(11) LOOM_CASE(12) LOOM_CASE(13) LOOM_CASE(14)
+        LOOM_CASE(15) LOOM_CASE(16) LOOM_CASE(17) LOOM_CASE(18) LOOM_CASE(19)
+        LOOM_CASE(20) LOOM_CASE(21) LOOM_CASE(22) LOOM_CASE(23) LOOM_CASE(24)
+        LOOM_CASE(25) LOOM_CASE(26) LOOM_CASE(27) LOOM_CASE(28) LOOM_CASE(29)
+        LOOM_CASE(30) LOOM_CASE(31) LOOM_CASE(32) LOOM_CASE(33) LOOM_CASE(34)
+        LOOM_CASE(35) LOOM_CASE(36) LOOM_CASE(37) LOOM_CASE(38) LOOM_CASE(39)
+        LOOM_CASE(40) LOOM_CASE(41) LOOM_CASE(42) LOOM_CASE(43) LOOM_CASE(44)
+        LOOM_CASE(45) LOOM_CASE(46) LOOM_CASE(47) LOOM_CASE(48) LOOM_CASE(49)
+        LOOM_CASE(50) LOOM_CASE(51) LOOM_CASE(52) LOOM_CASE(53) LOOM_CASE(54)
+        LOOM_CASE(55) LOOM_CASE(56) LOOM_CASE(57) LOOM_CASE(58) LOOM_CASE(59)
+        LOOM_CASE(60) LOOM_CASE(61) LOOM_CASE(62) LOOM_CASE(63)
+    default: return NONE;
+    }
+}
+#undef LOOM_CASE
+
+/// @brief Most populated block of the bit set @p set (ascending, strict >).
+LOOM_DEVICE std::uint32_t
+major_of_set(std::uint64_t set, std::uint32_t pc, std::uint32_t& pops)
+{
+    std::uint32_t best = NONE, bp = 0;
+    while(set)
+    {
+        const std::uint32_t b = static_cast<std::uint32_t>(__builtin_ctzll(set));
+        set &= set - 1;
+        const std::uint32_t p = popc(ballot(pc == b));
+        if(p > bp)
+        {
+            bp   = p;
+            best = b;
+        }
+    }
+    pops = bp;
+    return best;
+}
+
+/// @brief Most populated block among all live lanes (greedy groups).
+LOOM_DEVICE std::uint32_t
+major_present(std::uint32_t pc, std::uint32_t& pops)
+{
+    std::uint64_t live = ballot(pc != HALT);
+    std::uint32_t best = NONE, bp = 0;
+    while(live)
+    {
+        const int           l = __builtin_ctzll(live);
+        const std::uint32_t b = uniform(__builtin_amdgcn_readlane(pc, l));
+        const std::uint64_t m = ballot(pc == b);
+        live &= ~m;
+        const std::uint32_t p = popc(m);
+        if(p > bp || (p == bp && b < best))
+        {
+            bp   = p;
+            best = b;
+        }
+    }
+    pops = bp;
+    return best;
+}
+
+/// @brief Execute the wave while the chosen block has more than @p thr lanes.
+/// With @p force the first choice runs regardless of the threshold, which
+/// guarantees progress for every claimed batch.  Returns when no block has
+/// more than @p thr lanes (thr = 0: when every lane halted).
+template <typename K>
+LOOM_DEVICE void
+run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, bool force, WaveCtx& w)
+{
+    std::uint64_t pending = 0;
+    std::uint32_t pops    = 0;
+    std::uint32_t cur     = major_present(pc, pops);
+    if(!(pops > thr || (force && pops > 0))) return;
+    for(;;)
+    {
+        while(cur != NONE)
+        {
+            cur = dispatch<K>(cur, r, pc, pending, thr, w);
+            if(w.disp > w.max_steps)
+            {
+                w.over = true;
+                return;
+            }
+        }
+        pops = 0;
+        if(pending) cur = major_of_set(pending, pc, pops);
+        if(!pops) cur = major_present(pc, pops);
+        if(pops <= thr) return;
+    }
+}
+
+template <typename K>
+LOOM_DEVICE void
+flush_stats(const WaveCtx& w, const CtlArgs& c)
+{
+    if(c.stats && lane_id() == 0)
+    {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_OPS]), w.act);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_SLOT_OPS]), w.slots);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_DISPATCHES]), w.disp);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_THREAD_STEPS]), w.tsteps);
+    }
+}
+
+LOOM_DEVICE void
+raise_errors(std::uint32_t lane_err, bool over, const CtlArgs& c)
+{
+    std::uint32_t e = lane_err;
+    if(over) e |= ERR_STEP_BUDGET;
+    // OR-reduce via ballots of each bit (few bits)
+    std::uint32_t we = 0;
+    for(std::uint32_t b = 0; b < 3; ++b)
+        if(ballot((e >> b) & 1u)) we |= 1u << b;
+    if(we && c.err && lane_id() == 0) atomicOr(c.err, we);
+}
+
+// ---------------------------------------------------------------------------
+// compact form
+// ---------------------------------------------------------------------------
+/// @brief Predicated wave64 kernel: each wave runs 64 consecutive threads to
+/// completion (reference compact_compiler, simd.hpp:3742-4014).
+template <typename K>
+__global__ __launch_bounds__(256) void
+compact_kernel(IoArgs io, CtlArgs c)
+{
+    const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool          valid = tid < c.n;
+    Regs<K>             r;
+    std::uint32_t       pc = HALT;
+    if(valid)
+    {
+        load_inputs<K>(r, io.in, tid);
+        pc = 0;
+    }
+    WaveCtx w;
+    w.max_steps = c.max_steps;
+    run_to_threshold<K>(r, pc, 0, true, w);
+    if(valid) store_outputs<K>(r, io.out, tid);
+    flush_stats<K>(w, c);
+    raise_errors(w.err, w.over, c);
+}
+
+// ---------------------------------------------------------------------------
+// hybrid form: queue layout
+// ---------------------------------------------------------------------------
+/// @brief Byte layout of the regroup queues of kernel @p K: label b owns a
+/// region of `qcap` rows; its columns are the thread index (u32) followed by
+/// the registers live at b, each a contiguous array of `qcap` elements.
+template <typename K>
+struct QueueLayout
+{
+    static constexpr const auto& T = ir::table_of<K>;
+
+    /// row bytes of each label and their prefix
+    static constexpr auto rows = []
+    {
+        std::array<std::uint32_t, 65> pre{};
+        for(std::size_t b = 0; b < ir::table_of<K>.nb; ++b)
+        {
+            std::uint32_t w = 4;
+            for(std::size_t s = 0; s < ir::table_of<K>.nr; ++s)
+                if(ir::table_of<K>.block[b].live_in.test(s)) w += ir::table_of<K>.reg[s].width;
+            pre[b + 1] = pre[b] + w;
+        }
+        return pre;
+    }();
+
+    /// total bytes per queue row set (multiply by qcap for one side)
+    static constexpr std::uint32_t total = rows[ir::table_of<K>.nb];
+
+    /// column prefix (bytes per row before slot @p s) inside label @p b
+    static constexpr std::uint32_t
+    col(std::size_t b, std::size_t s)
+    {
+        std::uint32_t w = 4;
+        for(std::size_t k = 0; k < s; ++k)
+            if(ir::table_of<K>.block[b].live_in.test(k)) w += ir::table_of<K>.reg[k].width;
+        return w;
+    }
+
+    static constexpr std::uint32_t
+    live_bytes(std::size_t b)
+    {
+        return rows[b + 1] - rows[b];
+    }
+};
+
+/// @brief Write this lane's thread @p tid (at label @p B) to row @p row.
+template <typename K, std::size_t B>
+LOOM_DEVICE void
+spill_row(const Regs<K>& r, std::uint32_t tid, char* q, std::uint64_t qcap, std::uint32_t row)
+{
+    using QL                = QueueLayout<K>;
+    char* base              = q + qcap * QL::rows[B];
+    reinterpret_cast<std::uint32_t*>(base)[row] = tid;
+    [&]<std::size_t... S>(std::index_sequence<S...>)
+    {
+        (
+            [&]
+            {
+                if constexpr(ir::table_of<K>.block[B].live_in.test(S))
+                {
+                    char* col = base + qcap * QL::col(B, S);
+                    if constexpr(ir::table_of<K>.reg[S].width == 8)
+                        reinterpret_cast<std::uint64_t*>(col)[row] = r.template raw<S>();
+                    else
+                        reinterpret_cast<std::uint32_t*>(col)[row] =
+                            static_cast<std::uint32_t>(r.template raw<S>());
+                }
+            }(),
+            ...);
+    }(std::make_index_sequence<ir::table_of<K>.nr>{});
+}
+
+/// @brief Read thread state at label @p B from row @p row into this lane.
+template <typename K, std::size_t B>
+LOOM_DEVICE std::uint32_t
+fill_row(Regs<K>& r, const char* q, std::uint64_t qcap, std::uint32_t row)
+{
+    using QL                = QueueLayout<K>;
+    const char* base        = q + qcap * QL::rows[B];
+    const std::uint32_t tid = reinterpret_cast<const std::uint32_t*>(base)[row];
+    [&]<std::size_t... S>(std::index_sequence<S...>)
+    {
+        (
+            [&]
+            {
+                if constexpr(ir::table_of<K>.block[B].live_in.test(S))
+                {
+                    const char* col = base + qcap * QL::col(B, S);
+                    if constexpr(ir::table_of<K>.reg[S].width == 8)
+                        r.template set_raw<S>(reinterpret_cast<const std::uint64_t*>(col)[row]);
+                    else
+                        r.template set_raw<S>(reinterpret_cast<const std::uint32_t*>(col)[row]);
+                }
+            }(),
+            ...);
+    }(std::make_index_sequence<ir::table_of<K>.nr>{});
+    return tid;
+}
+
+#define LOOM_LABEL_SWITCH(EXPR)                                            \
+    switch(label)                                                          \
+    {                                                                      \
+        LOOM_LC(0) LOOM_LC(1) LOOM_LC(2) LOOM_LC(3) LOOM_LC(4) LOOM_LC(5)  \
+        LOOM_LC(6) LOOM_LC(7) LOOM_LC(8) LOOM_LC(9) LOOM_LC(10) LOOM_LC(11) \
+        LOOM_LC(12) LOOM_LC(13) LOOM_LC(14) LOOM_LC(15) LOOM_LC(16)        \
+        LOOM_LC(17) LOOM_LC(18) LOOM_LC(19) LOOM_LC(20) LOOM_LC(21)        \
+        LOOM_LC(22) LOOM_LC(23) LOOM_LC(24) LOOM_LC(25) LOOM_LC(26)        \
+        LOOM_LC(27) LOOM_LC(28) LOOM_LC(29) LOOM_LC(30) LOOM_LC(31)        \
+        LOOM_LC(32) LOOM_LC(33) LOOM_LC(34) LOOM_LC(35) LOOM_LC(36)        \
+        LOOM_LC(37) LOOM_LC(38) LOOM_LC(39) LOOM_LC(40) LOOM_LC(41)        \
+        LOOM_LC(42) LOOM_LC(43) LOOM_LC(44) LOOM_LC(45) LOOM_LC(46)        \
+        LOOM_LC(47) LOOM_LC(48) LOOM_LC(49) LOOM_LC(50) LOOM_LC(51)        \
+        LOOM_LC(52) LOOM_LC(53) LOOM_LC(54) LOOM_LC(55) LOOM_LC(56)        \
+        LOOM_LC(57) LOOM_LC(58) LOOM_LC(59) LOOM_LC(60) LOOM_LC(61)        \
+        LOOM_LC(62) LOOM_LC(63)                                            \
+    default: break;                                                        \
+    }
+
+template <typename K>
+LOOM_DEVICE void
+spill_label(std::uint32_t label, const Regs<K>& r, std::uint32_t tid, char* q,
+            std::uint64_t qcap, std::uint32_t row)
+{
+#define LOOM_LC(b)                                                              \
+    case b:                                                                     \
+        if constexpr(b < ir::table_of<K>.nb)                                    \
+            spill_row<K, (b < ir::table_of<K>.nb ? b : 0)>(r, tid, q, qcap, row); \
+        break;
+    LOOM_LABEL_SWITCH(0)
+#undef LOOM_LC
+}
+
+template <typename K>
+LOOM_DEVICE std::uint32_t
+fill_label(std::uint32_t label, Regs<K>& r, const char* q, std::uint64_t qcap,
+           std::uint32_t row)
+{
+    std::uint32_t tid = 0;
+#define LOOM_LC(b)                                                                  \
+    case b:                                                                         \
+        if constexpr(b < ir::table_of<K>.nb)                                        \
+            tid = fill_row<K, (b < ir::table_of<K>.nb ? b : 0)>(r, q, qcap, row);   \
+        break;
+    LOOM_LABEL_SWITCH(0)
+#undef LOOM_LC
+    return tid;
+}
+
+// ---------------------------------------------------------------------------
+// hybrid form: pass kernel
+// ---------------------------------------------------------------------------
+/// @brief One hybrid pass.  Launched with one wave per workgroup and a grid
+/// sized to the resident wave count; every wave loops claiming batches until
+/// the pass has none left, so each wave exits after finite work.
+template <typename K>
+__global__ __launch_bounds__(64) void
+hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
+{
+    using QL = QueueLayout<K>;
+    HybridCtl* ctl            = h.ctl;
+    const std::uint32_t par   = uniform(ctl->parity);
+    const std::uint32_t nbq   = uniform(ctl->nbatches);
+    const char*         qin   = h.q[par];
+    char*               qout  = h.q[par ^ 1u];
+    std::uint32_t*      qcnt  = ctl->qcount[par ^ 1u];
+    const std::uint64_t n     = c.n;
+    const std::uint32_t lane  = lane_id();
+
+    WaveCtx w;
+    w.max_steps = c.max_steps;
+    std::uint64_t spill_rows = 0, spill_bytes = 0, fill_rows = 0, fill_bytes = 0,
+                  admitted = 0, refills = 0, batches = 0, spill_failed = 0;
+
+    Regs<K>       r;
+    std::uint32_t pc  = HALT;
+    std::uint32_t tid = 0;
+    bool          has = false;
+
+    for(;;)
+    {
+        // ---- claim a batch (the wave is empty here) ----
+        pc              = HALT;
+        has             = false;
+        std::uint32_t b = 0;
+        if(lane == 0) b = atomicAdd(&ctl->batch_next, 1u);
+        b = uniform(b);
+        if(b < nbq)
+        {
+            // label of batch b: last label whose prefix is <= b
+            std::uint32_t label = 0;
+            while(label + 1 < 64 && ctl->bprefix[label + 1] <= b) ++label;
+            label                   = uniform(label);
+            const std::uint32_t cnt = uniform(ctl->qcount[par][label]);
+            const std::uint32_t row0 = (b - uniform(ctl->bprefix[label])) * 64u;
+            const std::uint32_t rows = cnt - row0 < 64u ? cnt - row0 : 64u;
+            if(lane < rows)
+            {
+                tid = fill_label<K>(label, r, qin, h.qcap, row0 + lane);
+                pc  = label;
+                has = true;
+            }
+            fill_rows += rows;
+            fill_bytes += std::uint64_t(rows) * QL::live_bytes(label);
+        }
+        else
+        {
+            std::uint64_t k = 0;
+            if(lane == 0) k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next_input), 64ull);
+            k = uniform64(k);
+            if(k >= n) break;
+            const std::uint64_t t = k + lane;
+            if(t < n)
+            {
+                load_inputs<K>(r, io.in, t);
+                tid = static_cast<std::uint32_t>(t);
+                pc  = 0;
+                has = true;
+            }
+            admitted += (n - k < 64 ? n - k : 64);
+        }
+        ++batches;
+
+        // ---- execute, refill, spill ----
+        bool force = true;
+        for(;;)
+        {
+            run_to_threshold<K>(r, pc, c.thr, force, w);
+            force = false;
+            if(w.over) break;
+            if(has && pc == HALT)
+            {
+                store_outputs<K>(r, io.out, tid);
+                has = false;
+            }
+            const std::uint64_t live = ballot(has);
+            if(!live) break;
+            if(c.refill)
+            {
+                const std::uint64_t empty = ~live;
+                std::uint64_t       k     = 0;
+                if(lane == 0)
+                    k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next_input),
+                                  static_cast<unsigned long long>(popc(empty)));
+                k = uniform64(k);
+                if(k < n && empty)
+                {
+                    const std::uint64_t t = k + rank_in(empty);
+                    if(!has && t < n)
+                    {
+                        load_inputs<K>(r, io.in, t);
+                        tid = static_cast<std::uint32_t>(t);
+                        pc  = 0;
+                        has = true;
+                    }
+                    admitted += (n - k < popc(empty) ? n - k : popc(empty));
+                    ++refills;
+                    force = true;
+                    continue;
+                }
+            }
+            // spill survivors into the queues of their labels
+            std::uint64_t todo = live;
+            while(todo)
+            {
+                const int           l     = __builtin_ctzll(todo);
+                const std::uint32_t label = uniform(__builtin_amdgcn_readlane(pc, l));
+                const std::uint64_t m     = ballot(has && pc == label);
+                todo &= ~m;
+                const std::uint32_t cntm = popc(m);
+                std::uint32_t       base = 0;
+                if(lane == 0) base = atomicAdd(&qcnt[label], cntm);
+                base = uniform(base);
+                const std::uint32_t row = base + rank_in(m);
+                if(has && pc == label && row < h.qcap)
+                {
+                    spill_label<K>(label, r, tid, qout, h.qcap, row);
+                    has = false;
+                }
+                const std::uint32_t ok = base >= h.qcap ? 0u :
+                                         (h.qcap - base < cntm ? std::uint32_t(h.qcap - base) : cntm);
+                spill_rows += ok;
+                spill_bytes += std::uint64_t(ok) * QL::live_bytes(label);
+                spill_failed += cntm - ok;
+            }
+            if(!ballot(has)) break;
+            // a queue was full: finish the remaining lanes in place
+            run_to_threshold<K>(r, pc, 0, true, w);
+            if(has && pc == HALT)
+            {
+                store_outputs<K>(r, io.out, tid);
+                has = false;
+            }
+            break;
+        }
+        if(w.over) break;
+    }
+    flush_stats<K>(w, c);
+    if(c.stats && lane == 0)
+    {
+        auto* s = reinterpret_cast<unsigned long long*>(c.stats);
+        atomicAdd(&s[ST_SPILL_ROWS], spill_rows);
+        atomicAdd(&s[ST_SPILL_BYTES], spill_bytes);
+        atomicAdd(&s[ST_FILL_ROWS], fill_rows);
+        atomicAdd(&s[ST_FILL_BYTES], fill_bytes);
+        atomicAdd(&s[ST_ADMITTED], admitted);
+        atomicAdd(&s[ST_REFILLS], refills);
+        atomicAdd(&s[ST_BATCHES], batches);
+        atomicAdd(&s[ST_SPILL_FAILED], spill_failed);
+    }
+    raise_errors(w.err, w.over, c);
+}
+
+/// @brief Between passes: turn the queues written by the finished pass into
+/// the batches of the next one (clamped to capacity), clear the other side
+/// and raise `done` when no thread is queued and no input remains.  With
+/// @p first set it initialises the control block.  One wave, lane 0 works.
+inline __global__ __launch_bounds__(64) void
+hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
+                    std::uint64_t n, std::uint32_t first)
+{
+    if(threadIdx.x != 0) return;
+    if(first)
+    {
+        ctl->next_input = 0;
+        ctl->batch_next = 0;
+        ctl->nbatches   = 0;
+        ctl->parity     = 0;
+        ctl->live       = 0;
+        ctl->passes     = 0;
+        ctl->done       = n == 0;
+        for(int s = 0; s < 2; ++s)
+            for(int l = 0; l < 64; ++l) ctl->qcount[s][l] = 0;
+        for(int l = 0; l < 65; ++l) ctl->bprefix[l] = 0;
+        return;
+    }
+    const std::uint32_t out  = ctl->parity ^ 1u;
+    std::uint32_t       live = 0, pre = 0;
+    for(std::uint32_t l = 0; l < 64; ++l)
+    {
+        std::uint32_t cnt = l < nb ? ctl->qcount[out][l] : 0u;
+        if(cnt > qcap) cnt = static_cast<std::uint32_t>(qcap);
+        ctl->qcount[out][l] = cnt;
+        ctl->bprefix[l]     = pre;
+        pre += (cnt + 63u) / 64u;
+        live += cnt;
+    }
+    ctl->bprefix[64] = pre;
+    for(std::uint32_t l = 0; l < 64; ++l) ctl->qcount[out ^ 1u][l] = 0;
+    ctl->parity     = out;
+    ctl->nbatches   = pre;
+    ctl->batch_next = 0;
+    ctl->live       = live;
+    ctl->passes += 1;
+    ctl->done = live == 0 && ctl->next_input >= n;
+}
+} // namespace loom::hipvm

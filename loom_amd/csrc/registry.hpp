@@ -1,0 +1,123 @@
+// -*- C++ -*-
+/// @file registry.hpp
+/// @brief Registration of lowered kernels with the runtime.
+///
+/// A translation unit compiled for gfx950 lowers a kernel and instantiates its
+/// device kernels with `LOOM_HIP_REGISTER("name", K)`.  The entry records the
+/// kernel table summary and launcher functions; the C ABI finds kernels by
+/// index or name.
+#pragma once
+
+#include "device/sched.hpp"
+
+#include <loom_amd.h>
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+namespace loom::hipvm
+{
+struct KernelEntry
+{
+    loom_kernel_info info{};
+    hipError_t (*compact)(const IoArgs&, const CtlArgs&, hipStream_t) = nullptr;
+    hipError_t (*hybrid_pass)(const IoArgs&, const CtlArgs&, const HybridArgs&,
+                              unsigned grid, hipStream_t)           = nullptr;
+    /// resident 64-thread workgroups per CU of the hybrid pass kernel
+    int (*hybrid_occupancy)()                                       = nullptr;
+    /// registers and scratch of the device kernels (0: compact, 1: hybrid)
+    int (*kernel_attrs)(int which, int* vgprs, int* sgprs, int* scratch) = nullptr;
+};
+
+/// @brief Add @p e to the process-wide registry; returns true.
+bool register_kernel(const KernelEntry& e);
+
+template <typename K>
+hipError_t
+launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
+{
+    if(c.n == 0) return hipSuccess;
+    const std::uint64_t blocks = (c.n + 255) / 256;
+    hipLaunchKernelGGL(compact_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                       io, c);
+    return hipGetLastError();
+}
+
+template <typename K>
+hipError_t
+launch_hybrid_pass(const IoArgs& io, const CtlArgs& c, const HybridArgs& h, unsigned grid,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(hybrid_pass_kernel<K>, dim3(grid), dim3(64), 0, s, io, c, h);
+    return hipGetLastError();
+}
+
+template <typename K>
+int
+hybrid_occupancy()
+{
+    int nblk = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K>, 64, 0) !=
+       hipSuccess)
+        return 1;
+    return nblk > 0 ? nblk : 1;
+}
+
+template <typename K>
+int
+kernel_attrs(int which, int* vgprs, int* sgprs, int* scratch)
+{
+    hipFuncAttributes a{};
+    const hipError_t  e =
+        which == 0 ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(compact_kernel<K>)) :
+                     hipFuncGetAttributes(&a, reinterpret_cast<const void*>(hybrid_pass_kernel<K>));
+    if(e != hipSuccess) return static_cast<int>(e);
+    if(vgprs) *vgprs = a.numRegs;
+    if(sgprs) *sgprs = 0;
+    if(scratch) *scratch = static_cast<int>(a.localSizeBytes);
+    return 0;
+}
+
+template <typename K>
+KernelEntry
+make_entry(const char* name)
+{
+    constexpr const auto& T = ir::table_of<K>;
+    static_assert(T.nin <= kMaxIO && T.nout <= kMaxIO, "too many kernel inputs/outputs");
+    KernelEntry e;
+    std::strncpy(e.info.name, name, sizeof(e.info.name) - 1);
+    e.info.hash        = T.hash;
+    e.info.n_inputs    = static_cast<std::int32_t>(T.nin);
+    e.info.n_outputs   = static_cast<std::int32_t>(T.nout);
+    for(std::size_t k = 0; k < T.nin; ++k)
+        e.info.input_type[k] = static_cast<std::int32_t>(T.in_ty[k]);
+    for(std::size_t k = 0; k < T.nout; ++k)
+        e.info.output_type[k] = static_cast<std::int32_t>(T.out_ty[k]);
+    e.info.n_blocks        = static_cast<std::int32_t>(T.nb);
+    e.info.n_registers     = static_cast<std::int32_t>(T.nr);
+    e.info.index_bytes     = T.index_bytes;
+    e.info.state_size      = static_cast<std::uint32_t>(T.state_size);
+    e.info.queue_row_bytes = QueueLayout<K>::total;
+    for(std::size_t b = 0; b < T.nb; ++b)
+    {
+        e.info.block_ops[b]        = T.block[b].ops;
+        e.info.block_live_bytes[b] = QueueLayout<K>::live_bytes(b);
+    }
+    e.compact          = &launch_compact<K>;
+    e.hybrid_pass      = &launch_hybrid_pass<K>;
+    e.hybrid_occupancy = &hybrid_occupancy<K>;
+    e.kernel_attrs     = &kernel_attrs<K>;
+    return e;
+}
+} // namespace loom::hipvm
+
+#define LOOM_REG_CAT(a, b) LOOM_REG_CAT_I(a, b)
+#define LOOM_REG_CAT_I(a, b) a##b
+#define LOOM_HIP_REGISTER_I(LINE, NAME, ...)                              \
+    static const bool LOOM_REG_CAT(loom_registered_, LINE) =               \
+        ::loom::hipvm::register_kernel(::loom::hipvm::make_entry<__VA_ARGS__>(NAME))
+
+/// @brief Lower kernel type `...` and register it under @p NAME.
+#define LOOM_HIP_REGISTER(NAME, ...) LOOM_HIP_REGISTER_I(__COUNTER__, NAME, __VA_ARGS__)

@@ -1,0 +1,496 @@
+// -*- C++ -*-
+/// @file runtime.cpp
+/// @brief The C ABI (include/loom_amd.h): device set-up, kernel registry,
+/// device memory, and the compact / hybrid run drivers.
+///
+/// One HIP stream per process carries all work (default priority).  Hybrid
+/// runs keep a grow-only workspace (control block and two queue sides) per
+/// kernel, so repeated runs do not allocate.
+#include "registry.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace loom::hipvm
+{
+namespace
+{
+std::vector<KernelEntry>&
+registry()
+{
+    static std::vector<KernelEntry> r;
+    return r;
+}
+
+thread_local std::string g_error;
+
+int
+fail(int code, const char* fmt, ...)
+{
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define LOOM_HIP(expr)                                                              \
+    do                                                                              \
+    {                                                                               \
+        const hipError_t e_ = (expr);                                               \
+        if(e_ != hipSuccess)                                                        \
+            return fail(static_cast<int>(e_), "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while(0)
+
+struct Device
+{
+    bool          ready = false;
+    int           id    = 0;
+    int           cus   = 0;
+    hipStream_t   stream{};
+    std::uint32_t* err   = nullptr; // device error flags
+    std::uint64_t* stats = nullptr; // device counters
+    std::uint32_t* host_err = nullptr;   // pinned
+    std::uint64_t* host_stats = nullptr; // pinned
+    HybridCtl*     host_ctl = nullptr;   // pinned mirror of a control block
+    hipEvent_t     ev[4]{};
+    std::vector<hipEvent_t> pass_ev;
+};
+
+Device g_dev;
+std::mutex g_mutex;
+
+struct Workspace
+{
+    HybridCtl*    ctl   = nullptr;
+    char*         q[2]  = {nullptr, nullptr};
+    std::size_t   bytes = 0; // per side
+};
+
+std::vector<Workspace> g_ws; // per kernel index
+
+struct HostCols
+{
+    std::vector<void*>       ptr;
+    std::vector<std::size_t> bytes;
+};
+HostCols g_hostcols;
+
+int
+ensure_device()
+{
+    if(!g_dev.ready) return fail(LOOM_E_NODEVICE, "loom_init has not been called");
+    return 0;
+}
+
+int
+type_size(int t)
+{
+    switch(t)
+    {
+    case LOOM_U8:
+    case LOOM_I8: return 1;
+    case LOOM_U16:
+    case LOOM_I16: return 2;
+    case LOOM_U32:
+    case LOOM_I32:
+    case LOOM_F32: return 4;
+    default: return 8;
+    }
+}
+
+int
+check_kernel(int k)
+{
+    if(k < 0 || k >= static_cast<int>(registry().size()))
+        return fail(LOOM_E_KERNEL, "unknown kernel index %d", k);
+    return 0;
+}
+
+IoArgs
+make_io(const KernelEntry& e, const void* const* in, void* const* out)
+{
+    IoArgs io{};
+    for(int k = 0; k < e.info.n_inputs; ++k) io.in[k] = in[k];
+    for(int k = 0; k < e.info.n_outputs; ++k) io.out[k] = out[k];
+    return io;
+}
+
+int
+finish_stats(loom_run_stats* st, const loom_params* p)
+{
+    LOOM_HIP(hipMemcpyAsync(g_dev.host_err, g_dev.err, sizeof(std::uint32_t),
+                            hipMemcpyDeviceToHost, g_dev.stream));
+    if(p && p->collect_stats)
+        LOOM_HIP(hipMemcpyAsync(g_dev.host_stats, g_dev.stats, sizeof(std::uint64_t) * kNumStats,
+                                hipMemcpyDeviceToHost, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    if(st)
+    {
+        st->errors = *g_dev.host_err;
+        if(p && p->collect_stats)
+            for(std::size_t i = 0; i < kNumStats; ++i) st->counters[i] = g_dev.host_stats[i];
+    }
+    if(*g_dev.host_err)
+        return fail(LOOM_E_RUNTIME, "kernel raised error flags 0x%x", *g_dev.host_err);
+    return 0;
+}
+
+int
+run_compact(const KernelEntry& e, const IoArgs& io, CtlArgs c, loom_run_stats* st)
+{
+    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
+    LOOM_HIP(e.compact(io, c, g_dev.stream));
+    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    float ms = 0;
+    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
+    if(st)
+    {
+        st->ms_total      = ms;
+        st->ms_main       = ms;
+        st->main_launches = 1;
+        st->passes        = 1;
+        st->grid_waves    = static_cast<std::uint32_t>((c.n + 63) / 64);
+    }
+    return 0;
+}
+
+int
+run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const loom_params* p,
+           loom_run_stats* st)
+{
+    // persistent grid: resident 64-lane workgroups
+    const int occ = e.hybrid_occupancy();
+    int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
+    wpc           = std::max(1, std::min(wpc, occ));
+    unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
+    const std::uint64_t need_waves = (c.n + 63) / 64;
+    if(need_waves < grid) grid = static_cast<unsigned>(std::max<std::uint64_t>(1, need_waves));
+
+    // queue capacity: rows per label, a multiple of 64
+    std::uint64_t qcap = p && p->queue_capacity ? p->queue_capacity :
+                                                  std::uint64_t(grid) * 64ull * 4ull;
+    qcap = std::min<std::uint64_t>(qcap, std::max<std::uint64_t>(c.n, 64));
+    qcap = (qcap + 63) / 64 * 64;
+    if(qcap > 0xFFFFFFC0ull) return fail(LOOM_E_CAPACITY, "queue capacity too large");
+
+    if(g_ws.size() < registry().size()) g_ws.resize(registry().size());
+    Workspace&        ws   = g_ws[kidx];
+    const std::size_t side = std::size_t(qcap) * e.info.queue_row_bytes;
+    if(!ws.ctl) LOOM_HIP(hipMalloc(&ws.ctl, sizeof(HybridCtl)));
+    if(ws.bytes < side)
+    {
+        for(auto& q: ws.q)
+            if(q) LOOM_HIP(hipFree(q));
+        for(auto& q: ws.q) LOOM_HIP(hipMalloc(&q, side));
+        ws.bytes = side;
+    }
+    HybridArgs h;
+    h.ctl  = ws.ctl;
+    h.q[0] = ws.q[0];
+    h.q[1] = ws.q[1];
+    h.qcap = qcap;
+
+    const std::uint32_t nb = static_cast<std::uint32_t>(e.info.n_blocks);
+    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
+    hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream, ws.ctl, nb, qcap,
+                       c.n, 1u);
+    LOOM_HIP(hipGetLastError());
+
+    std::uint32_t passes = 0, check_every = 1;
+    constexpr std::uint32_t max_passes = 1u << 20;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    std::size_t ev_used = 0;
+    auto next_ev = [&]() -> hipEvent_t
+    {
+        if(ev_used == g_dev.pass_ev.size())
+        {
+            hipEvent_t ev{};
+            if(hipEventCreate(&ev) != hipSuccess) return nullptr;
+            g_dev.pass_ev.push_back(ev);
+        }
+        return g_dev.pass_ev[ev_used++];
+    };
+    for(;;)
+    {
+        for(std::uint32_t k = 0; k < check_every; ++k)
+        {
+            hipEvent_t a = next_ev(), b = next_ev();
+            if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
+            LOOM_HIP(hipEventRecord(a, g_dev.stream));
+            LOOM_HIP(e.hybrid_pass(io, c, h, grid, g_dev.stream));
+            LOOM_HIP(hipEventRecord(b, g_dev.stream));
+            evs.emplace_back(a, b);
+            hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream, ws.ctl,
+                               nb, qcap, c.n, 0u);
+            LOOM_HIP(hipGetLastError());
+            ++passes;
+        }
+        LOOM_HIP(hipMemcpyAsync(g_dev.host_ctl, ws.ctl, sizeof(HybridCtl), hipMemcpyDeviceToHost,
+                                g_dev.stream));
+        LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+        if(g_dev.host_ctl->done) break;
+        if(passes >= max_passes) return fail(LOOM_E_RUNTIME, "hybrid run exceeded %u passes", max_passes);
+        check_every = std::min<std::uint32_t>(check_every * 2, 8);
+    }
+    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    float total = 0;
+    LOOM_HIP(hipEventElapsedTime(&total, g_dev.ev[0], g_dev.ev[1]));
+    double main_ms = 0;
+    for(auto& [a, b]: evs)
+    {
+        float ms = 0;
+        LOOM_HIP(hipEventElapsedTime(&ms, a, b));
+        main_ms += ms;
+    }
+    if(st)
+    {
+        st->ms_total      = total;
+        st->ms_main       = main_ms;
+        st->main_launches = static_cast<std::uint32_t>(evs.size());
+        st->passes        = passes;
+        st->grid_waves    = grid;
+    }
+    return 0;
+}
+} // namespace
+
+bool
+register_kernel(const KernelEntry& e)
+{
+    registry().push_back(e);
+    return true;
+}
+} // namespace loom::hipvm
+
+using namespace loom::hipvm;
+
+extern "C" {
+
+const char*
+loom_last_error(void)
+{
+    return g_error.c_str();
+}
+
+int
+loom_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(g_dev.ready) return device == g_dev.id ? 0 : fail(LOOM_E_ARG, "already initialised");
+    int count = 0;
+    if(hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(LOOM_E_NODEVICE, "no HIP device");
+    if(device < 0 || device >= count) return fail(LOOM_E_ARG, "device %d out of range", device);
+    LOOM_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop{};
+    LOOM_HIP(hipGetDeviceProperties(&prop, device));
+    g_dev.id  = device;
+    g_dev.cus = prop.multiProcessorCount;
+    LOOM_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    LOOM_HIP(hipMalloc(&g_dev.err, sizeof(std::uint32_t)));
+    LOOM_HIP(hipMalloc(&g_dev.stats, sizeof(std::uint64_t) * kNumStats));
+    LOOM_HIP(hipHostMalloc(&g_dev.host_err, sizeof(std::uint32_t), hipHostMallocDefault));
+    LOOM_HIP(hipHostMalloc(&g_dev.host_stats, sizeof(std::uint64_t) * kNumStats, hipHostMallocDefault));
+    LOOM_HIP(hipHostMalloc(&g_dev.host_ctl, sizeof(HybridCtl), hipHostMallocDefault));
+    for(auto& ev: g_dev.ev) LOOM_HIP(hipEventCreate(&ev));
+    g_dev.ready = true;
+    return 0;
+}
+
+int
+loom_device_info(int* compute_units, int* wave_size, size_t* global_mem, char* name,
+                 size_t name_len)
+{
+    if(int r = ensure_device()) return r;
+    hipDeviceProp_t prop{};
+    LOOM_HIP(hipGetDeviceProperties(&prop, g_dev.id));
+    if(compute_units) *compute_units = prop.multiProcessorCount;
+    if(wave_size) *wave_size = prop.warpSize;
+    if(global_mem) *global_mem = prop.totalGlobalMem;
+    if(name && name_len)
+    {
+        std::strncpy(name, prop.gcnArchName, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return 0;
+}
+
+int
+loom_kernel_count(void)
+{
+    return static_cast<int>(registry().size());
+}
+
+int
+loom_kernel_info_get(int index, loom_kernel_info* out)
+{
+    if(int r = check_kernel(index)) return r;
+    if(!out) return fail(LOOM_E_ARG, "null output");
+    *out = registry()[index].info;
+    return 0;
+}
+
+int
+loom_kernel_find(const char* name)
+{
+    if(!name) return -1;
+    for(std::size_t i = 0; i < registry().size(); ++i)
+        if(std::strcmp(registry()[i].info.name, name) == 0) return static_cast<int>(i);
+    fail(LOOM_E_KERNEL, "no kernel named %s", name);
+    return -1;
+}
+
+int
+loom_malloc(void** ptr, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    if(!ptr) return fail(LOOM_E_ARG, "null pointer");
+    LOOM_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int
+loom_free(void* ptr)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipFree(ptr));
+    return 0;
+}
+
+int
+loom_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_memset(void* dst, int value, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipMemsetAsync(dst, value, bytes, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_synchronize(void)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, uint64_t n,
+         const loom_params* params, loom_run_stats* stats)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(int r = check_kernel(kernel)) return r;
+    const KernelEntry& e = registry()[kernel];
+    if(n > 0xFFFFFFFFull) return fail(LOOM_E_CAPACITY, "at most 2^32-1 threads per run");
+    if(n && (!inputs || !outputs)) return fail(LOOM_E_ARG, "null column array");
+    for(int k = 0; n && k < e.info.n_inputs; ++k)
+        if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
+    for(int k = 0; n && k < e.info.n_outputs; ++k)
+        if(!outputs[k]) return fail(LOOM_E_ARG, "output column %d is null", k);
+    if(params && params->threshold > 63) return fail(LOOM_E_ARG, "threshold must be < 64");
+    if(stats) *stats = loom_run_stats{};
+    const IoArgs io = n ? make_io(e, inputs, outputs) : IoArgs{};
+    CtlArgs      c;
+    c.n         = n;
+    c.max_steps = params && params->max_steps ? params->max_steps : (1ull << 40);
+    c.thr       = params ? params->threshold : 0;
+    c.refill    = params ? params->refill : 1;
+    c.err       = g_dev.err;
+    c.stats     = params && params->collect_stats ? g_dev.stats : nullptr;
+    LOOM_HIP(hipMemsetAsync(g_dev.err, 0, sizeof(std::uint32_t), g_dev.stream));
+    if(c.stats)
+        LOOM_HIP(hipMemsetAsync(g_dev.stats, 0, sizeof(std::uint64_t) * kNumStats, g_dev.stream));
+    int r = 0;
+    if(mode == LOOM_MODE_COMPACT)
+        r = run_compact(e, io, c, stats);
+    else if(mode == LOOM_MODE_HYBRID)
+        r = run_hybrid(kernel, e, io, c, params, stats);
+    else
+        return fail(LOOM_E_ARG, "unknown mode %d", mode);
+    if(r) return r;
+    return finish_stats(stats, params);
+}
+
+int
+loom_run_host(int kernel, int mode, const void* const* inputs, void* const* outputs, uint64_t n,
+              const loom_params* params, loom_run_stats* stats)
+{
+    if(int r = ensure_device()) return r;
+    if(int r = check_kernel(kernel)) return r;
+    const KernelEntry& e  = registry()[kernel];
+    const int          ni = e.info.n_inputs, no = e.info.n_outputs;
+    std::vector<void*> din(ni), dout(no);
+    {
+        std::lock_guard<std::mutex> lk(g_mutex);
+        auto& hc = g_hostcols;
+        if(hc.ptr.size() < static_cast<std::size_t>(ni + no))
+        {
+            hc.ptr.resize(ni + no, nullptr);
+            hc.bytes.resize(ni + no, 0);
+        }
+        for(int k = 0; k < ni + no; ++k)
+        {
+            const int         t     = k < ni ? e.info.input_type[k] : e.info.output_type[k - ni];
+            const std::size_t bytes = std::max<std::size_t>(1, n * type_size(t));
+            if(hc.bytes[k] < bytes)
+            {
+                if(hc.ptr[k]) LOOM_HIP(hipFree(hc.ptr[k]));
+                LOOM_HIP(hipMalloc(&hc.ptr[k], bytes));
+                hc.bytes[k] = bytes;
+            }
+            (k < ni ? din[k] : dout[k - ni]) = hc.ptr[k];
+        }
+        for(int k = 0; k < ni && n; ++k)
+            LOOM_HIP(hipMemcpyAsync(din[k], inputs[k], n * type_size(e.info.input_type[k]),
+                                    hipMemcpyHostToDevice, g_dev.stream));
+    }
+    if(int r = loom_run(kernel, mode, din.data(), dout.data(), n, params, stats)) return r;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    for(int k = 0; k < no && n; ++k)
+        LOOM_HIP(hipMemcpyAsync(outputs[k], dout[k], n * type_size(e.info.output_type[k]),
+                                hipMemcpyDeviceToHost, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+} // extern "C"

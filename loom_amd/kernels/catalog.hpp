@@ -1,0 +1,19 @@
+// -*- C++ -*-
+/// @file catalog.hpp
+/// @brief The built-in kernels: one list, expanded by the gfx950 library
+/// (registration) and by the CPU oracle, so both address kernels by the same
+/// names.  X(name, kernel type).
+#pragma once
+
+#include "programs.hpp"
+
+#define LOOM_AMD_CATALOG_CORE(X)                                      \
+    X("fib_u64", loom_amd::programs::fib<std::uint64_t>)              \
+    X("fib_u32", loom_amd::programs::fib<std::uint32_t>)              \
+    X("collatz_u64", loom_amd::programs::collatz<std::uint64_t>)      \
+    X("factorial_u64", loom_amd::programs::factorial<std::uint64_t>)  \
+    X("jump4_u32", loom_amd::programs::jump4<std::uint32_t>)          \
+    X("int_mix_u64", loom_amd::programs::int_mix<std::uint64_t>)      \
+    X("int_mix_u32", loom_amd::programs::int_mix<std::uint32_t>)      \
+    X("fp_mix_f32", loom_amd::programs::fp_mix<float>)                \
+    X("fp_mix_f64", loom_amd::programs::fp_mix<double>)

@@ -1,0 +1,44 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+All randomness is splitmix64 (Steele, Lea & Flood 2014) over a counter, so
+inputs are identical on every machine and in every language that restates
+it; no std distributions (implementation-defined) are used.  The default
+seed is 0x4C4F4F4D ("LOOM").
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEED = 0x4C4F4F4D
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """n outputs of splitmix64 started at state `seed`, skipping `offset`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_int(seed: int, n: int, lo: int, hi: int) -> np.ndarray:
+    """Integers uniform in [lo, hi] (inclusive) by 32-bit multiply-shift of
+    the high word (Lemire's reduction without rejection)."""
+    span = hi - lo + 1
+    assert 0 < span <= 1 << 32
+    top = splitmix64(seed, n) >> np.uint64(32)
+    return (np.uint64(lo) + ((top * np.uint64(span)) >> np.uint64(32))).astype(np.uint64)
+
+
+def fib_inputs(n: int = 1 << 20, seed: int = SEED) -> np.ndarray:
+    """fib config: n ∈ [0, 90] (F(90) < 2^64)."""
+    return uniform_int(seed, n, 0, 90)
+
+
+def collatz_inputs(n: int = 1 << 24, seed: int = SEED + 1) -> np.ndarray:
+    """Collatz config: start values uniform in [1, 2^32)."""
+    return uniform_int(seed, n, 1, (1 << 32) - 1)

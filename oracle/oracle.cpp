@@ -1,0 +1,198 @@
+// -*- C++ -*-
+/// @file oracle.cpp
+/// @brief ORACLE (test infrastructure only): C entry points of the scalar
+/// restatement, for the parity tests and the CPU baseline timing.  Never
+/// linked into or called by the product library.
+#include "bse.hpp"
+
+#include "../loom_amd/kernels/catalog.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using namespace loom_oracle;
+
+namespace
+{
+struct Entry
+{
+    const char* name;
+    Program (*build)();
+};
+
+#define LOOM_ORACLE_ENTRY(NAME, ...) Entry{NAME, &build_program<__VA_ARGS__>},
+const Entry g_entries[] = {LOOM_AMD_CATALOG_CORE(LOOM_ORACLE_ENTRY)};
+#undef LOOM_ORACLE_ENTRY
+
+constexpr int g_count = sizeof(g_entries) / sizeof(g_entries[0]);
+
+std::vector<Program>&
+programs()
+{
+    static std::vector<Program> p = []
+    {
+        std::vector<Program> v;
+        for(const auto& e: g_entries) v.push_back(e.build());
+        return v;
+    }();
+    return p;
+}
+
+void
+run_range(const Program& p, const void* const* in, void* const* out, std::uint64_t begin,
+          std::uint64_t end, std::atomic<std::uint64_t>& executed, std::atomic<std::uint32_t>& err)
+{
+    std::vector<std::uint64_t> R(p.reg_name.size());
+    std::uint64_t              ex = 0;
+    std::uint32_t              e  = 0;
+    for(std::uint64_t t = begin; t < end; ++t)
+    {
+        std::fill(R.begin(), R.end(), 0);
+        for(std::size_t k = 0; k < p.in_reg.size(); ++k)
+        {
+            const int b = ty_bytes(p.in_ty[k]);
+            std::uint64_t v = 0;
+            std::memcpy(&v, static_cast<const char*>(in[k]) + t * b, b);
+            R[p.in_reg[k]] = v;
+        }
+        ex += run_thread(p, R.data(), e);
+        for(std::size_t k = 0; k < p.out_reg.size(); ++k)
+        {
+            const int b = ty_bytes(p.out_ty[k]);
+            std::memcpy(static_cast<char*>(out[k]) + t * b, &R[p.out_reg[k]], b);
+        }
+    }
+    executed += ex;
+    err |= e;
+}
+} // namespace
+
+extern "C" {
+
+int
+oracle_kernel_count(void)
+{
+    return g_count;
+}
+
+const char*
+oracle_kernel_name(int k)
+{
+    return k >= 0 && k < g_count ? g_entries[k].name : nullptr;
+}
+
+int
+oracle_kernel_find(const char* name)
+{
+    for(int k = 0; k < g_count; ++k)
+        if(std::strcmp(g_entries[k].name, name) == 0) return k;
+    return -1;
+}
+
+/// Types of inputs/outputs (codes as loom_type) and register count.
+int
+oracle_kernel_io(int k, int* nin, int* nout, int* in_types, int* out_types, int* nregs)
+{
+    if(k < 0 || k >= g_count) return 1;
+    const Program& p = programs()[k];
+    *nin             = static_cast<int>(p.in_ty.size());
+    *nout            = static_cast<int>(p.out_ty.size());
+    for(std::size_t i = 0; i < p.in_ty.size(); ++i) in_types[i] = static_cast<int>(p.in_ty[i]);
+    for(std::size_t i = 0; i < p.out_ty.size(); ++i) out_types[i] = static_cast<int>(p.out_ty[i]);
+    if(nregs) *nregs = static_cast<int>(p.reg_name.size());
+    return 0;
+}
+
+/// Run threads [begin, begin+count) of host columns; `threads` CPU workers.
+int
+oracle_run(int k, const void* const* in, void* const* out, uint64_t begin, uint64_t count,
+           int threads, uint64_t* executed, uint32_t* errors)
+{
+    if(k < 0 || k >= g_count) return 1;
+    const Program&             p = programs()[k];
+    std::atomic<std::uint64_t> ex{0};
+    std::atomic<std::uint32_t> er{0};
+    threads = std::max(1, threads);
+    if(threads == 1 || count < 1024)
+        run_range(p, in, out, begin, begin + count, ex, er);
+    else
+    {
+        std::vector<std::thread> pool;
+        const std::uint64_t      chunk = (count + threads - 1) / threads;
+        for(int t = 0; t < threads; ++t)
+        {
+            const std::uint64_t b = begin + chunk * t;
+            const std::uint64_t e = std::min(begin + count, b + chunk);
+            if(b >= e) break;
+            pool.emplace_back([&, b, e] { run_range(p, in, out, b, e, ex, er); });
+        }
+        for(auto& th: pool) th.join();
+    }
+    if(executed) *executed = ex;
+    if(errors) *errors = er;
+    return 0;
+}
+
+int
+oracle_parse_f32(const char* s, float* v)
+{
+    auto r = lit::parse<float>(s);
+    if(!r) return 1;
+    *v = *r;
+    return 0;
+}
+
+int
+oracle_parse_f64(const char* s, double* v)
+{
+    auto r = lit::parse<double>(s);
+    if(!r) return 1;
+    *v = *r;
+    return 0;
+}
+
+int
+oracle_parse_u64(const char* s, uint64_t* v)
+{
+    auto r = lit::parse<std::uint64_t>(s);
+    if(!r) return 1;
+    *v = *r;
+    return 0;
+}
+
+int
+oracle_parse_i32(const char* s, int32_t* v)
+{
+    auto r = lit::parse<std::int32_t>(s);
+    if(!r) return 1;
+    *v = *r;
+    return 0;
+}
+
+float
+oracle_sin_f32(float x)
+{
+    return sin_poly(x);
+}
+
+float
+oracle_cos_f32(float x)
+{
+    return cos_poly(x);
+}
+
+double
+oracle_sin_f64(double x)
+{
+    return sin_poly(x);
+}
+
+double
+oracle_cos_f64(double x)
+{
+    return cos_poly(x);
+}
+}

@@ -1,0 +1,71 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so): the checker
+of the parity tests.  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+PATH = ROOT / "oracle" / "_build" / "liboracle.so"
+DTYPES = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64, 4: np.int8, 5: np.int16,
+          6: np.int32, 7: np.int64, 8: np.float32, 9: np.float64}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not PATH.exists():
+            raise RuntimeError(f"{PATH} missing: run `python -m loom_amd.build`")
+        L = C.CDLL(str(PATH))
+        L.oracle_kernel_find.argtypes = [C.c_char_p]
+        L.oracle_kernel_name.restype = C.c_char_p
+        L.oracle_kernel_io.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]
+        L.oracle_run.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                 C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint32)]
+        for nm, t in (("f32", C.c_float), ("f64", C.c_double)):
+            getattr(L, f"oracle_parse_{nm}").argtypes = [C.c_char_p, C.POINTER(t)]
+            for f in ("sin", "cos"):
+                fn = getattr(L, f"oracle_{f}_{nm}")
+                fn.argtypes = [t]
+                fn.restype = t
+        L.oracle_parse_u64.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
+        L.oracle_parse_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
+        _lib = L
+    return _lib
+
+
+def io(name):
+    L = lib()
+    k = L.oracle_kernel_find(name.encode())
+    if k < 0:
+        raise KeyError(name)
+    ni, no, nr = C.c_int(), C.c_int(), C.c_int()
+    it, ot = (C.c_int * 16)(), (C.c_int * 16)()
+    L.oracle_kernel_io(k, C.byref(ni), C.byref(no), it, ot, C.byref(nr))
+    return k, [DTYPES[it[i]] for i in range(ni.value)], [DTYPES[ot[i]] for i in range(no.value)]
+
+
+def run(name, inputs, threads=8):
+    """Outputs of kernel `name` on host columns `inputs` (oracle)."""
+    k, itys, otys = io(name)
+    ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
+    n = len(ins[0])
+    outs = [np.zeros(n, t) for t in otys]
+    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
+    op = (C.c_void_p * 16)(*[a.ctypes.data for a in outs])
+    ex, er = C.c_uint64(), C.c_uint32()
+    rc = lib().oracle_run(k, ip, op, 0, n, threads, C.byref(ex), C.byref(er))
+    assert rc == 0
+    return outs, er.value
+
+
+def parse(kind, text):
+    v = {"f32": C.c_float, "f64": C.c_double, "u64": C.c_uint64, "i32": C.c_int32}[kind]()
+    rc = getattr(lib(), f"oracle_parse_{kind}")(text.encode(), C.byref(v))
+    return None if rc else v.value

@@ -1,0 +1,230 @@
+"""The oracle pinned against closed forms, brute force and exact arithmetic.
+
+The reference commits no golden vectors; its tests pin kernels with
+closed-form validators (reference tests/programs.hpp:288-389) and sorter
+invariants.  These tests pin the CPU oracle the same way before it is used
+as the checker of the GPU backend.
+"""
+import math
+import struct
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from loom_amd import workloads
+from tests import oracle_lib as O
+
+M64 = (1 << 64) - 1
+
+
+def fib(n):
+    a, b = 0, 1
+    for _ in range(n):
+        a, b = b, a + b
+    return a
+
+
+def collatz(x):
+    c = 0
+    while x > 1:
+        x = x // 2 if x % 2 == 0 else 3 * x + 1
+        assert x < 1 << 64
+        c += 1
+    return c
+
+
+def test_fib_all_inputs():
+    n = np.arange(91, dtype=np.uint64)
+    (c,), err = O.run("fib_u64", [n])
+    assert err == 0
+    assert [int(v) for v in c] == [fib(i) for i in range(91)]
+
+
+def test_fib_u32_wraps():
+    n = np.arange(60, dtype=np.uint32)
+    (c,), _ = O.run("fib_u32", [n])
+    assert [int(v) for v in c] == [fib(i) % (1 << 32) for i in range(60)]
+
+
+def test_collatz_brute_force():
+    x = workloads.collatz_inputs(3000)
+    x = np.concatenate([x, np.array([1, 2, 3, 27, 97, 871, (1 << 32) - 1], np.uint64)])
+    (s,), err = O.run("collatz_u64", [x])
+    assert err == 0
+    assert [int(v) for v in s] == [collatz(int(v)) for v in x]
+    assert int(s[-4]) == 111 and int(s[-3]) == 118 and int(s[-2]) == 178
+
+
+def test_factorial():
+    a = np.arange(30, dtype=np.uint64)
+    (r,), _ = O.run("factorial_u64", [a])
+    assert [int(v) for v in r] == [(math.factorial(i) & M64) if i > 1 else 1 for i in range(30)]
+
+
+def test_jump4_and_out_of_range_index():
+    i = np.array([0, 1, 2, 3, 4, 5, 1 << 31], np.uint32)
+    (o,), _ = O.run("jump4_u32", [i])
+    assert list(map(int, o)) == [1, 3, 7, 11, 4, 5, 1 << 31]
+
+
+def int_mix_expected(a, b, bits):
+    """int_mix with the op contract: modular arithmetic, x/0 = all ones,
+    x%0 = x, shift counts modulo the promoted width (32 or 64)."""
+    m = (1 << bits) - 1
+    w = 32 if bits <= 32 else 64
+    s = (a + b) & m
+    d = (a - b) & m
+    mul = (a * b) & m
+    q = m if b == 0 else a // b
+    r = a if b == 0 else a % b
+    x1, x2, x3, x4 = a & b, a | b, a ^ b, (~a) & m
+    l = ((a << (b % w)) & ((1 << w) - 1)) & m
+    h = a >> (b % w)
+    mn, mx = min(a, b), max(a, b)
+    ng = (-a) & m
+    la, lo, ln = int(bool(a) and bool(b)), int(bool(a) or bool(b)), int(not a)
+    c1, c2, c3 = int(a < b), int(a >= b), int(a != b)
+    sc = [10, 0x20, 3, 4][c1]
+    p1 = int(bool(c2) and a < b)
+    p2 = int(bool(c1) or a >= b)
+    x1 = x1 ^ x2 ^ x3 ^ x4
+    s = (s + l + h) & m
+    d = (d + mn + mx) & m
+    mul = (mul + ng) & m
+    q = (q + r) & m
+    la = (la + lo + ln + c1 + c2 + c3 + sc + p1 + p2) & m
+    return [s, d, mul, q, x1, la]
+
+
+@pytest.mark.parametrize("name,bits", [("int_mix_u64", 64), ("int_mix_u32", 32)])
+def test_int_mix(name, bits):
+    dt = np.uint64 if bits == 64 else np.uint32
+    raw = workloads.splitmix64(7, 4000)
+    a = (raw & np.uint64((1 << bits) - 1)).astype(dt)
+    b = (workloads.splitmix64(8, 4000) & np.uint64((1 << bits) - 1)).astype(dt)
+    # small and edge operands: zero divisors, shift counts around the width
+    b[:200] = np.arange(200) % 70
+    a[:5] = [0, 1, (1 << bits) - 1, 1 << (bits - 1), 12345]
+    outs, err = O.run(name, [a, b])
+    assert err == 0
+    for i in range(len(a)):
+        exp = int_mix_expected(int(a[i]), int(b[i]), bits)
+        got = [int(o[i]) for o in outs]
+        assert got == exp, (i, int(a[i]), int(b[i]), got, exp)
+
+
+# ---- exact float32 arithmetic for the fp_mix check ---------------------------------
+def f32(x):
+    return float(np.float32(x))
+
+
+def round_f32(q: Fraction) -> float:
+    """Correctly rounded (nearest-even) float32 of a rational."""
+    if q == 0:
+        return 0.0
+    x = float(q)  # nearest double, then fix the float32 rounding exactly
+    c = np.float32(x)
+    lo, hi = np.nextafter(c, np.float32(-np.inf)), np.nextafter(c, np.float32(np.inf))
+    best = min((lo, c, hi), key=lambda v: (abs(Fraction(float(v)) - q),
+                                           struct.unpack("<I", struct.pack("<f", v))[0] & 1))
+    return float(best)
+
+
+def fma32(x, y, z):
+    return round_f32(Fraction(x) * Fraction(y) + Fraction(z))
+
+
+def test_fp_mix_f32_exact():
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-900, 900, 800).astype(np.float32)
+    y = rng.uniform(-50, 50, 800).astype(np.float32)
+    y[::7] = np.float32(0.25)
+    x[:4] = [0.0, -0.0, 1e-3, 4e9]
+    outs, err = O.run("fp_mix_f32", [x, y])
+    assert err == 0
+    a_o, b_o, cs_o, sn_o, u_o, sum_o = outs
+    for i in range(len(x)):
+        X, Y = float(x[i]), float(y[i])
+        a = f32(f32(X * Y) + X)
+        b = f32(X / Y)
+        c = fma32(X, Y, b)
+        d = fma32(X, Y, -a)
+        g1 = f32(math.sqrt(abs(X)))
+        g2 = f32(1.0 / f32(math.sqrt(g1))) if g1 != 0 else math.inf
+        r1 = f32(math.copysign(math.floor(abs(b) + 0.5), b) + math.floor(b))
+        r3 = f32(math.ceil(b) + math.trunc(b))
+        mn = Y if Y < X else X
+        mx = Y if X < Y else X
+        cs = math.copysign(abs(X), Y)
+        sel = X < Y
+        pick = mn if sel else mx
+        k = [0.5, 1.25, -300.0][int(sel)]
+        a = f32(f32(a + c) + d)
+        b = f32(f32(f32(b + g2) + r1) + r3)
+        cs = f32(f32(cs + pick) + k)
+        u = 0 if X <= 0 else min(int(X), (1 << 32) - 1)
+        s = f32(a + b)
+        assert float(a_o[i]) == a, i
+        assert float(b_o[i]) == b or (math.isnan(b) and math.isnan(b_o[i])), i
+        assert float(cs_o[i]) == cs, i
+        assert int(u_o[i]) == u, i
+        assert float(sum_o[i]) == s or (math.isnan(s) and math.isnan(sum_o[i])), i
+        ang = f32(X * f32(0.001))
+        assert abs(float(sn_o[i]) - math.sin(ang) * math.cos(ang)) < 2e-6 or abs(ang) > math.pi
+
+
+# ---- sin/cos polynomials ---------------------------------------------------------
+def test_sin_cos_polynomials_close_to_libm():
+    L = O.lib()
+    xs = np.linspace(-math.pi, math.pi, 20001)
+    e32 = max(max(abs(L.oracle_sin_f32(float(np.float32(x))) - math.sin(float(np.float32(x)))),
+                  abs(L.oracle_cos_f32(float(np.float32(x))) - math.cos(float(np.float32(x)))))
+              for x in xs)
+    e64 = max(max(abs(L.oracle_sin_f64(x) - math.sin(x)), abs(L.oracle_cos_f64(x) - math.cos(x)))
+              for x in xs)
+    assert e32 < 1e-6  # float32 evaluation; measured 4.8e-7
+    assert e64 < 2e-15  # measured 7.4e-16
+    assert L.oracle_sin_f32(0.0) == 0.0 and L.oracle_cos_f64(0.0) == 1.0
+
+
+# ---- literals --------------------------------------------------------------------------
+@pytest.mark.parametrize("text,value", [
+    ("0", 0.0), ("1", 1.0), ("0.5", 0.5), ("-0.5", -0.5), ("0.25", 0.25), ("1.25", 1.25),
+    ("-3e2", -300.0), ("1e3", 1000.0), ("100", 100.0), ("+2", 2.0), ("16.5", 16.5),
+    ("1e", 1.0),      # empty exponent is ignored (string.hpp:343)
+    ("e1", 0.0),      # no digits before the exponent: a literal 0 (string.hpp:330)
+    (".", 0.0),
+])
+def test_literal_exact_cases(text, value):
+    assert O.parse("f32", text) == value
+    assert O.parse("f64", text) == value
+
+
+@pytest.mark.parametrize("text", ["x", "-", "", "1.x", "cmp", "0x1"])
+def test_literal_rejects_names(text):
+    assert O.parse("f32", text) is None
+
+
+def test_literal_inexact_within_ulps_of_strtof():
+    # the reference parser is not correctly rounded; it stays within a few
+    # ulp of the correctly rounded value for the literals kernels use
+    for t in ["0.1", "0.093073", "6.283185307179586", "0.0001", "1e-12", "1e-6",
+              "3.14159265", "0.999", "81.6", "1e5", "681.33", "0.042612", "0.5135",
+              "295.6", "1.5", "0.75", "12"]:
+        got = np.float32(O.parse("f32", t))
+        want = np.float32(float(t))
+        ulps = abs(int(got.view(np.int32)) - int(want.view(np.int32)))
+        assert ulps <= 4, (t, got, want)
+        got64 = O.parse("f64", t)
+        ulps64 = abs(int(np.float64(got64).view(np.int64)) - int(np.float64(float(t)).view(np.int64)))
+        assert ulps64 <= 8, (t, got64)
+
+
+def test_integer_literals():
+    assert O.parse("u64", "0x20") == 32
+    assert O.parse("u64", "0b11") == 3
+    assert O.parse("u64", "18446744073709551615") == M64
+    assert O.parse("u64", "-1") is None          # unsigned: no sign accepted
+    assert O.parse("i32", "-0x10") == -16
+    assert O.parse("u64", "0x") is None
