@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark: path-traced samples/s of the hybrid (regrouping) gfx950 backend
+against the predicated compact backend on the smallpt 9-sphere scene,
+1024x768 at 64 spp, one thread per sample.
+
+A step renders one whole frame (50,331,648 sample threads) with inputs
+resident in HBM.  The JSON line reports the hybrid throughput as `value`,
+the compact throughput and active-lane fractions beside it, the roofline of
+the dominant kernel (hybrid pass kernel, HIP events around each launch) and
+the CPU baseline (the oracle's direct C++ path tracer on the host cores, a
+bounded sample of the same frame).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("path-trace samples/sec + active-lane % (hybrid vs compact GPU); "
+          "p99 req latency")
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published
+W, H, SPP = 1024, 768, 64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-compact", action="store_true", help="skip the compact comparison")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the hybrid pass kernel measured with rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by a copy kernel) and
+    committed as a short summary; None if it is absent."""
+    p = ROOT / "profiles" / "smallpt9_hybrid_traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline():
+    """The oracle's direct C++ path tracer (compiled scalar restatement),
+    one thread per host core, on the first 1/8 of the frame."""
+    from tests import oracle_lib
+    from loom_amd import workloads
+    n = W * H * SPP // 8
+    threads = min(os.cpu_count() or 1, 16)
+    sid = np.arange(n, dtype=np.uint32)
+    job = np.full(n, workloads.pt_job(W, H, SPP), np.uint32)
+    t = time.perf_counter()
+    oracle_lib.smallpt9_direct(sid, job, threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} samples (rows 0-95) of the {W}x{H}x{SPP} frame, "
+                      f"direct C++ restatement of the kernel, {threads} threads, "
+                      f"{dt:.2f} s wall"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported in this build "
+              f"(the path shards by frame or by sample range; see DESIGN.md)", file=sys.stderr)
+        return 2
+    import loom_amd
+    from loom_amd import abi, workloads, autotune
+
+    loom_amd.init(0)
+    k = loom_amd.kernel("smallpt9_f32")
+    n = W * H * SPP
+    ins = [loom_amd.DeviceArray.from_numpy(x) for x in workloads.pt_inputs(W, H, SPP)]
+    outs = [loom_amd.DeviceArray(n, np.float32) for _ in range(3)]
+
+    hyb = autotune.best_params("smallpt9_f32")
+    hp = loom_amd.make_params(mode=abi.MODE_HYBRID, **hyb)
+    cp = loom_amd.make_params(mode=abi.MODE_COMPACT)
+
+    def steps(mode, params, count):
+        last = None
+        t0 = time.perf_counter()
+        for _ in range(count):
+            last = loom_amd.run(k, mode, ins, outs, params=params)
+        abi.check(abi.load().loom_synchronize(), "sync")
+        return (time.perf_counter() - t0) * 1e3 / max(count, 1), last
+
+    steps(abi.MODE_HYBRID, hp, a.warmup)
+    h_ms, h_last = steps(abi.MODE_HYBRID, hp, a.steps)
+    h_img = [o.numpy() for o in outs]
+
+    result = {
+        "metric": METRIC,
+        "value": n / (h_ms / 1e3),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": h_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"smallpt9 path tracer {W}x{H} at {SPP} spp, one thread per "
+                               f"sample ({n} threads), 9-sphere scene, max depth 32",
+                   "width": W, "height": H, "spp": SPP, "samples": n,
+                   "backend": "hybrid", "hybrid_params": hyb},
+        "active_lane_fraction": h_last.active_lane_fraction,
+    }
+
+    if not a.no_compact:
+        steps(abi.MODE_COMPACT, cp, 1)
+        c_ms, c_last = steps(abi.MODE_COMPACT, cp, a.steps)
+        c_img = [o.numpy() for o in outs]
+        same = all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                   for x, y in zip(h_img, c_img))
+        result["compact"] = {"value": n / (c_ms / 1e3), "ms_per_step": c_ms,
+                             "active_lane_fraction": c_last.active_lane_fraction,
+                             "reconvergence": "successor-major (reference policy)"}
+        result["hybrid_over_compact"] = c_ms / h_ms
+        result["outputs_identical"] = same
+
+    # roofline of the dominant kernel: the hybrid pass kernel
+    launches = max(h_last.main_launches, 1)
+    launch_ms = h_last.ms_main / launches
+    flops = h_last.counters[abi.C_ACTIVE_FLOPS] / launches
+    achieved = flops / (launch_ms / 1e3) / 1e12
+    result["roofline"] = {
+        "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profile(),
+        "kernel": "hybrid_pass_kernel<smallpt9_f32, topo>",
+        "launch_ms": launch_ms, "launches_per_step": launches,
+        "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
+                "(fma = 2) per launch over the launch time, against the FP32 vector peak"}
+
+    if not a.no_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

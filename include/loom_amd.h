@@ -69,7 +69,8 @@ enum loom_counter
     LOOM_C_ADMITTED,       /* threads admitted from the input */
     LOOM_C_REFILLS,        /* in-wave refill events */
     LOOM_C_BATCHES,        /* batches claimed */
-    LOOM_C_SPILL_FAILED    /* lanes kept because a queue was full */
+    LOOM_C_SPILL_FAILED,   /* lanes kept because a queue was full */
+    LOOM_C_ACTIVE_FLOPS    /* Σ popcount(EXEC)·flops(B), fma = 2 */
 };
 
 /* error flag bits of loom_run_stats.errors */
@@ -87,12 +88,20 @@ typedef struct loom_kernel_info
     uint32_t queue_row_bytes; /* Σ over labels of 4 + live-in bytes */
     uint16_t block_ops[LOOM_MAX_BLOCKS];        /* #Compute + 1 */
     uint32_t block_live_bytes[LOOM_MAX_BLOCKS]; /* 4 + live-in bytes */
+    uint16_t block_flops[LOOM_MAX_BLOCKS];      /* floating-point ops */
 } loom_kernel_info;
 
 typedef struct loom_params
 {
-    uint32_t threshold;      /* hybrid regroup threshold in lanes (0..63) */
+    uint32_t threshold;      /* block-population threshold in lanes (0..63):
+                                a wave stops when its best block has no more
+                                (reference UTR, simd.hpp:3270-3281) */
+    uint32_t live_threshold; /* hybrid: stop when live lanes <= this (0..63) */
     uint32_t refill;         /* hybrid: admit new threads into free lanes */
+    uint32_t regroup;        /* hybrid: spill survivors to per-label queues */
+    uint32_t reconv;         /* reconvergence policy: 0 = successor-major
+                                (reference simd.hpp:3974-4005), 1 = topological
+                                (loop heads wait for their bodies) */
     uint32_t waves_per_cu;   /* hybrid persistent grid; 0 = occupancy limit */
     uint32_t collect_stats;  /* fill loom_run_stats.counters */
     uint64_t queue_capacity; /* hybrid rows per label queue; 0 = automatic */

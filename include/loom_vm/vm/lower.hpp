@@ -605,6 +605,7 @@ struct Block
     std::uint16_t i0 = 0, ni = 0;  ///< instruction range
     std::uint16_t s0 = 0, ns = 0;  ///< successor range (block index or halt16)
     std::uint16_t ops = 0;         ///< #Compute + 1: the cost unit of a step
+    std::uint16_t flops = 0;       ///< floating-point operations (fma = 2)
     Bits          use{}, def{}, live_in{};
 };
 
@@ -629,6 +630,14 @@ struct Table
     std::size_t ni = 0, no = 0, nt = 0, nc = 0, nb = 0, nsucc = 0, nr = 0,
                 nin = 0, nout = 0;
     std::size_t state_size = 0;  ///< Σ register widths (core.hpp:1027-1030)
+    /// scheduling order of the topological policy: blocks outside loops, then
+    /// loop blocks, each group in reverse post-order of a DFS from START
+    std::array<std::uint8_t, 64> order{};
+    std::array<std::uint8_t, 64> rank{};
+    /// loop headers (targets of DFS back edges) and their loop bodies
+    std::size_t                   nhead = 0;
+    std::array<std::uint8_t, 64>  head{};
+    std::array<std::uint64_t, 64> body{}; ///< block mask, header excluded
     Bits        outputs{};
     std::uint64_t hash = 0;
 };
@@ -774,6 +783,7 @@ lower()
         Block&            blk   = t.block[b];
         blk.i0                  = static_cast<std::uint16_t>(t.ni);
         std::size_t ncomp       = 0;
+        std::size_t flops       = 0;
         bool        last_jump   = false;
         for(std::size_t i = bstart[b]; i < s_end; ++i)
         {
@@ -803,6 +813,9 @@ lower()
             {
                 ++ncomp;
                 last_jump = false;
+                if(ty_float(s.ety) && s.op != Op::mov && s.op != Op::cmov &&
+                   s.op != Op::sct && s.op != Op::slt && s.op != Op::assert_fp)
+                    flops += (s.op == Op::fma || s.op == Op::fms) ? 2 : 1;
             }
             t.instr[t.ni++] = in;
         }
@@ -817,7 +830,8 @@ lower()
             t.instr[t.ni++]  = in;
         }
         blk.ni  = static_cast<std::uint16_t>(t.ni - blk.i0);
-        blk.ops = static_cast<std::uint16_t>(ncomp + 1);
+        blk.ops   = static_cast<std::uint16_t>(ncomp + 1);
+        blk.flops = static_cast<std::uint16_t>(flops);
         // successors: targets of the last instruction (straight-line: none)
         blk.s0 = static_cast<std::uint16_t>(t.nsucc);
         if(!t.straightline)
@@ -940,7 +954,82 @@ lower()
         }
     }
 
-    // ---- 6. structural hash -----------------------------------------------------
+    // ---- 6. reconvergence order and loops ----------------------------------
+    {
+        std::array<std::uint8_t, 64> state{}; // 0 new, 1 on stack, 2 done
+        std::array<std::uint8_t, 64> post{};
+        std::size_t                  npost = 0;
+        std::array<std::uint8_t, 64> stack{}, next{};
+        std::array<std::uint64_t, 64> back{}; // back[h]: sources of back edges to h
+        std::size_t sp = 0;
+        stack[sp++]    = 0;
+        state[0]       = 1;
+        while(sp)
+        {
+            const std::uint8_t b   = stack[sp - 1];
+            const Block&       blk = t.block[b];
+            if(next[b] < blk.ns)
+            {
+                const std::uint16_t s = t.succ[blk.s0 + next[b]++];
+                if(s == halt16) continue;
+                if(state[s] == 0)
+                {
+                    state[s]    = 1;
+                    stack[sp++] = static_cast<std::uint8_t>(s);
+                }
+                else if(state[s] == 1)
+                    back[s] |= std::uint64_t{1} << b;
+            }
+            else
+            {
+                state[b]      = 2;
+                post[npost++] = b;
+                --sp;
+            }
+        }
+        std::array<std::uint8_t, 64> rpo{};
+        std::size_t                  nr = 0;
+        for(std::size_t k = npost; k-- > 0;) rpo[nr++] = post[k];
+        for(std::size_t b = 0; b < nb; ++b)
+            if(state[b] == 0) rpo[nr++] = static_cast<std::uint8_t>(b); // unreachable
+        // natural loop bodies: blocks reaching a back-edge source without
+        // passing through the header
+        for(std::size_t h = 0; h < nb; ++h)
+        {
+            if(!back[h]) continue;
+            std::uint64_t body = 0, work = back[h] & ~(std::uint64_t{1} << h);
+            while(work)
+            {
+                std::size_t b = 0;
+                while(!((work >> b) & 1u)) ++b;
+                work &= ~(std::uint64_t{1} << b);
+                if((body >> b) & 1u) continue;
+                body |= std::uint64_t{1} << b;
+                for(std::size_t p = 0; p < nb; ++p) // predecessors of b
+                {
+                    const Block& pb = t.block[p];
+                    for(std::size_t k = 0; k < pb.ns; ++k)
+                        if(t.succ[pb.s0 + k] == b && p != h && !((body >> p) & 1u))
+                            work |= std::uint64_t{1} << p;
+                }
+            }
+            t.head[t.nhead]   = static_cast<std::uint8_t>(h);
+            t.body[t.nhead++] = body;
+        }
+        // scheduling order: blocks outside every loop first (each runs at
+        // most once per thread, so running them early frees lanes at loop
+        // exits), then loop blocks, both in reverse post-order
+        std::uint64_t in_loop = 0;
+        for(std::size_t k = 0; k < t.nhead; ++k)
+            in_loop |= t.body[k] | (std::uint64_t{1} << t.head[k]);
+        std::size_t r = 0;
+        for(int pass = 0; pass < 2; ++pass)
+            for(std::size_t k = 0; k < nr; ++k)
+                if((((in_loop >> rpo[k]) & 1u) != 0) == (pass == 1)) t.order[r++] = rpo[k];
+        for(std::size_t k = 0; k < nb; ++k) t.rank[t.order[k]] = static_cast<std::uint8_t>(k);
+    }
+
+    // ---- 7. structural hash -----------------------------------------------------
     std::uint64_t h = 0xCBF29CE484222325ull;
     h               = fnv(h, t.kname);
     for(std::size_t r = 0; r < t.nr; ++r)

@@ -63,6 +63,7 @@ class Kernel:
     queue_row_bytes: int
     block_ops: list = field(default_factory=list)
     block_live_bytes: list = field(default_factory=list)
+    block_flops: list = field(default_factory=list)
 
 
 def kernel(name_or_index) -> Kernel:
@@ -84,7 +85,8 @@ def kernel(name_or_index) -> Kernel:
         outputs=[TYPE_DTYPES[info.output_type[k]] for k in range(info.n_outputs)],
         n_blocks=nb, n_registers=info.n_registers, index_bytes=info.index_bytes,
         state_size=info.state_size, queue_row_bytes=info.queue_row_bytes,
-        block_ops=list(info.block_ops[:nb]), block_live_bytes=list(info.block_live_bytes[:nb]))
+        block_ops=list(info.block_ops[:nb]), block_live_bytes=list(info.block_live_bytes[:nb]),
+        block_flops=list(info.block_flops[:nb]))
 
 
 def kernels() -> list[Kernel]:
@@ -150,11 +152,19 @@ class RunResult:
         return self.counters[abi.C_ACTIVE_OPS] / slot if slot else float("nan")
 
 
-def make_params(threshold=0, refill=True, waves_per_cu=0, queue_capacity=0, max_steps=0,
-                stats=True) -> abi.Params:
+def make_params(threshold=0, live_threshold=0, refill=True, regroup=True, waves_per_cu=0,
+                queue_capacity=0, max_steps=0, stats=True, reconv=None,
+                mode=abi.MODE_COMPACT) -> abi.Params:
+    """Run parameters; `reconv` defaults to successor-major for the compact
+    form (the reference policy) and to the topological policy for hybrid."""
     p = abi.Params()
+    if reconv is None:
+        reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_TOPO
+    p.reconv = int(reconv)
     p.threshold = int(threshold)
+    p.live_threshold = int(live_threshold)
     p.refill = 1 if refill else 0
+    p.regroup = 1 if regroup else 0
     p.waves_per_cu = int(waves_per_cu)
     p.queue_capacity = int(queue_capacity)
     p.max_steps = int(max_steps)
@@ -173,7 +183,7 @@ def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = 
     lib = abi.load()
     if len(inputs) != len(k.inputs):
         raise ValueError(f"{k.name} takes {len(k.inputs)} inputs, got {len(inputs)}")
-    params = params or make_params()
+    params = params or make_params(mode=mode)
     st = abi.RunStats()
     device = isinstance(inputs[0], DeviceArray)
     if n is None:
@@ -212,7 +222,7 @@ class compact_function:
 
     def __init__(self, k, **params):
         self.kernel = kernel(k)
-        self.params = make_params(**params)
+        self.params = make_params(mode=self.mode, **params)
         self.last: RunResult | None = None
 
     def __call__(self, *inputs):
@@ -224,11 +234,14 @@ class hybrid_function(compact_function):
     """Regrouping persistent form (reference simd::hybrid_function<K>,
     src/vm/simd.hpp:4310-4317).  `threshold` is the regroup threshold in lanes
     (the reference UTRt·N, simd.hpp:4094-4098), `refill` admits new threads
-    into free lanes, `waves_per_cu` sizes the persistent grid."""
+    into free lanes when at most `live_threshold` lanes are live, `regroup`
+    spills survivors to per-label queues once no input is left,
+    `waves_per_cu` sizes the persistent grid."""
 
     mode = abi.MODE_HYBRID
 
-    def __init__(self, k, threshold=32, refill=True, waves_per_cu=0, queue_capacity=0,
-                 **params):
-        super().__init__(k, threshold=threshold, refill=refill, waves_per_cu=waves_per_cu,
+    def __init__(self, k, threshold=0, live_threshold=48, refill=True, regroup=True,
+                 waves_per_cu=0, queue_capacity=0, **params):
+        super().__init__(k, threshold=threshold, live_threshold=live_threshold, refill=refill,
+                         regroup=regroup, waves_per_cu=waves_per_cu,
                          queue_capacity=queue_capacity, **params)

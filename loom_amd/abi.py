@@ -18,10 +18,14 @@ NUM_COUNTERS = 16
 MODE_COMPACT = 0
 MODE_HYBRID = 1
 
+RECONV_SUCCESSOR_MAJOR = 0
+RECONV_TOPO = 1
+
 # loom_counter
 C_ACTIVE_OPS, C_SLOT_OPS, C_DISPATCHES, C_THREAD_STEPS = 0, 1, 2, 3
 C_SPILL_ROWS, C_SPILL_BYTES, C_FILL_ROWS, C_FILL_BYTES = 4, 5, 6, 7
 C_ADMITTED, C_REFILLS, C_BATCHES, C_SPILL_FAILED = 8, 9, 10, 11
+C_ACTIVE_FLOPS = 12
 
 ERR_ASSERT_FP = 1
 ERR_STEP_BUDGET = 2
@@ -42,13 +46,17 @@ class KernelInfo(C.Structure):
         ("queue_row_bytes", C.c_uint32),
         ("block_ops", C.c_uint16 * MAX_BLOCKS),
         ("block_live_bytes", C.c_uint32 * MAX_BLOCKS),
+        ("block_flops", C.c_uint16 * MAX_BLOCKS),
     ]
 
 
 class Params(C.Structure):
     _fields_ = [
         ("threshold", C.c_uint32),
+        ("live_threshold", C.c_uint32),
         ("refill", C.c_uint32),
+        ("regroup", C.c_uint32),
+        ("reconv", C.c_uint32),
         ("waves_per_cu", C.c_uint32),
         ("collect_stats", C.c_uint32),
         ("queue_capacity", C.c_uint64),

@@ -1,0 +1,97 @@
+"""Autotuner for the hybrid backend: picks the live-lane refill threshold,
+the regroup switch, the reconvergence policy and the persistent grid
+(waves per CU) by timing candidates on a sample of the real workload.
+
+    python -m loom_amd.autotune smallpt9_f32      # tunes, writes tuned.json
+
+Results are stored per kernel in loom_amd/tuned.json; `best_params` reads
+them (or returns the defaults) so benchmarks and batched launches start from
+measured settings.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+TUNED = Path(__file__).resolve().parent / "tuned.json"
+
+DEFAULT = {"live_threshold": 48, "regroup": False, "reconv": 1, "waves_per_cu": 0,
+           "threshold": 0, "refill": True}
+
+SPACE = {
+    "live_threshold": [32, 40, 48, 56],
+    "regroup": [False, True],
+    "reconv": [0, 1],
+    "waves_per_cu": [0, 12, 16],
+}
+
+
+def best_params(kernel_name: str) -> dict:
+    """Tuned hybrid parameters of `kernel_name` (defaults if never tuned)."""
+    if TUNED.exists():
+        data = json.loads(TUNED.read_text())
+        if kernel_name in data:
+            return {**DEFAULT, **data[kernel_name]["params"]}
+    return dict(DEFAULT)
+
+
+def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
+         verbose: bool = True) -> dict:
+    """Time every candidate of `space` on `inputs` (numpy columns, copied to
+    HBM once) and return the best parameters with their time."""
+    import loom_amd
+    from loom_amd import abi
+
+    loom_amd.init(0)
+    k = loom_amd.kernel(kernel_name)
+    n = len(inputs[0])
+    din = [loom_amd.DeviceArray.from_numpy(a) for a in inputs]
+    dout = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
+    space = space or SPACE
+    keys = list(space)
+    results = []
+    for combo in itertools.product(*(space[k2] for k2 in keys)):
+        cand = {**DEFAULT, **dict(zip(keys, combo))}
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=False, **cand)
+        loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p)  # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        results.append((ms, cand))
+        if verbose:
+            print(f"tune {kernel_name} {cand} {ms:.3f} ms", flush=True)
+    results.sort(key=lambda r: r[0])
+    return {"params": {k2: results[0][1][k2] for k2 in keys}, "ms": results[0][0],
+            "n": n, "candidates": len(results)}
+
+
+def save(kernel_name: str, entry: dict) -> None:
+    data = json.loads(TUNED.read_text()) if TUNED.exists() else {}
+    data[kernel_name] = entry
+    TUNED.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+
+
+def main(argv=None) -> int:
+    from loom_amd import workloads
+    argv = argv or sys.argv[1:]
+    name = argv[0] if argv else "smallpt9_f32"
+    if name.startswith("smallpt9"):
+        ins = workloads.pt_inputs(1024, 768, 8)       # a frame at 1/8 of the spp
+    elif name.startswith("collatz"):
+        ins = [workloads.collatz_inputs(1 << 22)]
+    else:
+        ins = [workloads.fib_inputs(1 << 20)]
+    entry = tune(name, ins)
+    save(name, entry)
+    print(json.dumps({name: entry}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

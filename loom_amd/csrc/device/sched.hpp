@@ -35,7 +35,7 @@
 namespace loom::hipvm
 {
 inline constexpr int         kMaxIO    = 16;
-inline constexpr std::size_t kNumStats = 12;
+inline constexpr std::size_t kNumStats = 13;
 
 /// @brief Kernel input/output columns (device pointers, structure of arrays).
 struct IoArgs
@@ -59,6 +59,7 @@ enum Stat : int
     ST_REFILLS,        ///< in-wave refill events
     ST_BATCHES,        ///< batches claimed by persistent waves
     ST_SPILL_FAILED,   ///< lanes kept because a queue was full
+    ST_ACTIVE_FLOPS,   ///< Σ popcount(EXEC)·flops(B): useful floating-point work
 };
 
 /// @brief Per-run control arguments.
@@ -66,16 +67,24 @@ struct CtlArgs
 {
     std::uint64_t  n         = 0;       ///< number of threads
     std::uint64_t  max_steps = 1ull << 40; ///< per-wave block dispatch budget
-    std::uint32_t  thr       = 0;       ///< regroup threshold (lanes)
+    std::uint32_t  thr       = 0;       ///< block-population threshold (lanes)
+    std::uint32_t  live_thr  = 0;       ///< hybrid: stop when live lanes <= this
     std::uint32_t  refill    = 1;       ///< hybrid: refill free lanes in-wave
+    std::uint32_t  regroup   = 1;       ///< hybrid: spill survivors to queues
     std::uint32_t* err       = nullptr; ///< error flags
     std::uint64_t* stats     = nullptr; ///< kNumStats counters or null
 };
 
+/// @brief Input partitions of a hybrid run: one per XCD, so that waves
+/// (dispatched round-robin over the 8 XCDs) mostly claim from their own
+/// counter and their own L2.
+inline constexpr std::uint32_t kRanges = 8;
+
 /// @brief Hybrid control block, resident in device memory.
 struct HybridCtl
 {
-    std::uint64_t next_input;     ///< next thread index to admit
+    std::uint64_t next[kRanges];  ///< next unadmitted thread of each range
+    std::uint64_t end[kRanges];   ///< end of each range
     std::uint32_t batch_next;     ///< batch claim counter of the pass
     std::uint32_t nbatches;       ///< queue batches of the pass
     std::uint32_t parity;         ///< in-queue side of the pass
@@ -139,7 +148,7 @@ uniform64(std::uint64_t v)
 /// @brief Per-wave scheduling state (wave-uniform values).
 struct WaveCtx
 {
-    std::uint64_t act = 0, slots = 0, disp = 0, tsteps = 0;
+    std::uint64_t act = 0, slots = 0, disp = 0, tsteps = 0, flops = 0;
     std::uint64_t max_steps = 0;
     std::uint32_t err = 0;   ///< lane-local error bits
     bool          over = false;
@@ -160,6 +169,7 @@ step(Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, W
     w.slots += 64ull * blk.ops;
     w.disp += 1;
     w.tsteps += n;
+    w.flops += std::uint64_t(n) * blk.flops;
     pending &= ~(1ull << B);
 
     if constexpr(blk.ns == 0)
@@ -272,18 +282,21 @@ major_present(std::uint32_t pc, std::uint32_t& pops)
     return best;
 }
 
-/// @brief Execute the wave while the chosen block has more than @p thr lanes.
-/// With @p force the first choice runs regardless of the threshold, which
-/// guarantees progress for every claimed batch.  Returns when no block has
-/// more than @p thr lanes (thr = 0: when every lane halted).
+/// @brief Execute the wave while the chosen block has more than @p thr lanes
+/// and more than @p live_thr lanes are live.  With @p force the first choice
+/// runs regardless of the thresholds, which guarantees progress for every
+/// claimed batch.  Returns when no block has more than @p thr lanes (thr = 0:
+/// when every lane halted) or when at most @p live_thr lanes remain live.
 template <typename K>
 LOOM_DEVICE void
-run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, bool force, WaveCtx& w)
+run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr,
+                 bool force, WaveCtx& w)
 {
     std::uint64_t pending = 0;
     std::uint32_t pops    = 0;
     std::uint32_t cur     = major_present(pc, pops);
     if(!(pops > thr || (force && pops > 0))) return;
+    if(!force && live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
     for(;;)
     {
         while(cur != NONE)
@@ -294,12 +307,157 @@ run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, bool force, W
                 w.over = true;
                 return;
             }
+            if(live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
         }
         pops = 0;
         if(pending) cur = major_of_set(pending, pc, pops);
         if(!pops) cur = major_present(pc, pops);
         if(pops <= thr) return;
     }
+}
+
+/// @brief Reconvergence policies.
+enum Policy : int
+{
+    /// reference SUCCESSOR_MAJOR threaded dispatch (simd.hpp:3974-4005)
+    POLICY_SUCCESSOR_MAJOR = 0,
+    /// lowest reverse-post-order block first; a loop header waits while any
+    /// lane is inside its loop body (reconvergence at the loop head)
+    POLICY_TOPO = 1,
+};
+
+/// @brief Mask of the blocks that hold live lanes.
+LOOM_DEVICE std::uint64_t
+present_blocks(std::uint32_t pc)
+{
+    std::uint64_t live = ballot(pc != HALT), pres = 0;
+    while(live)
+    {
+        const int           l = __builtin_ctzll(live);
+        const std::uint32_t b = uniform(__builtin_amdgcn_readlane(pc, l));
+        live &= ~ballot(pc == b);
+        pres |= 1ull << b;
+    }
+    return pres;
+}
+
+/// @brief Next block under POLICY_TOPO among the blocks of @p present.
+template <typename K>
+LOOM_DEVICE std::uint32_t
+topo_pick(std::uint64_t present)
+{
+    if(!present) return NONE;
+    std::uint64_t elig = present;
+    [&]<std::size_t... H>(std::index_sequence<H...>)
+    {
+        ((elig &= (present & ir::table_of<K>.body[H]) ? ~(1ull << ir::table_of<K>.head[H]) : ~0ull),
+         ...);
+    }(std::make_index_sequence<ir::table_of<K>.nhead>{});
+    if(!elig) elig = present;
+    std::uint32_t best = NONE;
+    [&]<std::size_t... R>(std::index_sequence<R...>)
+    {
+        (((elig >> ir::table_of<K>.order[R]) & 1u ? (best = ir::table_of<K>.order[R], true) : false) ||
+         ...);
+    }(std::make_index_sequence<ir::table_of<K>.nb>{});
+    return best;
+}
+
+/// @brief POLICY_TOPO step: run block @p B, update the present set from its
+/// successors, pick the next block.
+template <typename K, std::size_t B>
+LOOM_DEVICE std::uint32_t
+step_topo(Regs<K>& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
+{
+    constexpr ir::Block blk = ir::table_of<K>.block[B];
+    const std::uint64_t m   = ballot(pc == B);
+    if(pc == B) exec_block<K, B>(r, pc, w.err);
+    const std::uint32_t n = popc(m);
+    w.act += std::uint64_t(n) * blk.ops;
+    w.slots += 64ull * blk.ops;
+    w.disp += 1;
+    w.tsteps += n;
+    w.flops += std::uint64_t(n) * blk.flops;
+    present &= ~(1ull << B);
+    [&]<std::size_t... J>(std::index_sequence<J...>)
+    {
+        (
+            [&]
+            {
+                constexpr std::uint32_t s = ir::table_of<K>.succ[blk.s0 + J];
+                if constexpr(s != HALT)
+                    if(ballot(pc == s)) present |= 1ull << s;
+            }(),
+            ...);
+    }(std::make_index_sequence<blk.ns>{});
+    return topo_pick<K>(present);
+}
+
+#define LOOM_TCASE(b)                                                                     \
+    case b:                                                                               \
+        if constexpr(b < ir::table_of<K>.nb)                                              \
+            return step_topo<K, (b < ir::table_of<K>.nb ? b : 0)>(r, pc, present, w);     \
+        else                                                                              \
+            return NONE;
+
+template <typename K>
+LOOM_DEVICE std::uint32_t
+dispatch_topo(std::uint32_t cur, Regs<K>& r, std::uint32_t& pc, std::uint64_t& present,
+              WaveCtx& w)
+{
+    switch(cur)
+    {
+        LOOM_TCASE(0) LOOM_TCASE(1) LOOM_TCASE(2) LOOM_TCASE(3) LOOM_TCASE(4)
+        LOOM_TCASE(5) LOOM_TCASE(6) LOOM_TCASE(7) LOOM_TCASE(8) LOOM_TCASE(9)
+        LOOM_TCASE(10) LOOM_TCASE(11) LOOM_TCASE(12) LOOM_TCASE(13) LOOM_TCASE(14)
+        LOOM_TCASE(15) LOOM_TCASE(16) LOOM_TCASE(17) LOOM_TCASE(18) LOOM_TCASE(19)
+        LOOM_TCASE(20) LOOM_TCASE(21) LOOM_TCASE(22) LOOM_TCASE(23) LOOM_TCASE(24)
+        LOOM_TCASE(25) LOOM_TCASE(26) LOOM_TCASE(27) LOOM_TCASE(28) LOOM_TCASE(29)
+        LOOM_TCASE(30) LOOM_TCASE(31) LOOM_TCASE(32) LOOM_TCASE(33) LOOM_TCASE(34)
+        LOOM_TCASE(35) LOOM_TCASE(36) LOOM_TCASE(37) LOOM_TCASE(38) LOOM_TCASE(39)
+        LOOM_TCASE(40) LOOM_TCASE(41) LOOM_TCASE(42) LOOM_TCASE(43) LOOM_TCASE(44)
+        LOOM_TCASE(45) LOOM_TCASE(46) LOOM_TCASE(47) LOOM_TCASE(48) LOOM_TCASE(49)
+        LOOM_TCASE(50) LOOM_TCASE(51) LOOM_TCASE(52) LOOM_TCASE(53) LOOM_TCASE(54)
+        LOOM_TCASE(55) LOOM_TCASE(56) LOOM_TCASE(57) LOOM_TCASE(58) LOOM_TCASE(59)
+        LOOM_TCASE(60) LOOM_TCASE(61) LOOM_TCASE(62) LOOM_TCASE(63)
+    default: return NONE;
+    }
+}
+#undef LOOM_TCASE
+
+/// @brief POLICY_TOPO runner: execute until every lane halted or at most
+/// @p live_thr lanes are live (the first step runs regardless with @p force).
+template <typename K>
+LOOM_DEVICE void
+run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
+{
+    std::uint64_t present = present_blocks(pc);
+    std::uint32_t cur     = topo_pick<K>(present);
+    if(cur == NONE) return;
+    if(!force && live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
+    for(;;)
+    {
+        cur = dispatch_topo<K>(cur, r, pc, present, w);
+        if(w.disp > w.max_steps)
+        {
+            w.over = true;
+            return;
+        }
+        if(cur == NONE) return;
+        if(live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
+    }
+}
+
+/// @brief Run the wave under policy @p P (see run_to_threshold / run_topo).
+template <typename K, int P>
+LOOM_DEVICE void
+run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr, bool force,
+           WaveCtx& w)
+{
+    if constexpr(P == POLICY_TOPO)
+        run_topo<K>(r, pc, live_thr, force, w);
+    else
+        run_to_threshold<K>(r, pc, thr, live_thr, force, w);
 }
 
 template <typename K>
@@ -312,6 +470,7 @@ flush_stats(const WaveCtx& w, const CtlArgs& c)
         atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_SLOT_OPS]), w.slots);
         atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_DISPATCHES]), w.disp);
         atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_THREAD_STEPS]), w.tsteps);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_FLOPS]), w.flops);
     }
 }
 
@@ -332,7 +491,7 @@ raise_errors(std::uint32_t lane_err, bool over, const CtlArgs& c)
 // ---------------------------------------------------------------------------
 /// @brief Predicated wave64 kernel: each wave runs 64 consecutive threads to
 /// completion (reference compact_compiler, simd.hpp:3742-4014).
-template <typename K>
+template <typename K, int P>
 __global__ __launch_bounds__(256) void
 compact_kernel(IoArgs io, CtlArgs c)
 {
@@ -347,7 +506,7 @@ compact_kernel(IoArgs io, CtlArgs c)
     }
     WaveCtx w;
     w.max_steps = c.max_steps;
-    run_to_threshold<K>(r, pc, 0, true, w);
+    run_policy<K, P>(r, pc, 0, 0, true, w);
     if(valid) store_outputs<K>(r, io.out, tid);
     flush_stats<K>(w, c);
     raise_errors(w.err, w.over, c);
@@ -504,10 +663,67 @@ fill_label(std::uint32_t label, Regs<K>& r, const char* q, std::uint64_t qcap,
 // ---------------------------------------------------------------------------
 // hybrid form: pass kernel
 // ---------------------------------------------------------------------------
+/// @brief A wave's private reservoir of unadmitted thread indices.  Refills
+/// draw from it without atomics; it is topped up from the shared input
+/// ranges in guided chunks (large while much input remains, one batch near
+/// the end), preferring the range of the wave's XCD.  Exhausted ranges are
+/// remembered so that they are not polled again.
+struct Reservoir
+{
+    std::uint64_t base      = 0;
+    std::uint32_t count     = 0;
+    std::uint32_t exhausted = 0;   ///< bit r: range r has no input left
+    std::uint32_t chunk     = 512; ///< next claim size
+};
+
+/// @brief Take up to @p want consecutive thread indices (wave-uniform).
+LOOM_DEVICE std::uint32_t
+take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t want,
+           std::uint64_t& base)
+{
+    if(res.count == 0)
+    {
+        std::uint64_t b = 0, e = 0;
+        std::uint32_t got = 0, exh = res.exhausted;
+        if(lane_id() == 0)
+        {
+            for(std::uint32_t i = 0; i < kRanges && !got; ++i)
+            {
+                const std::uint32_t r = (home + i) % kRanges;
+                if((exh >> r) & 1u) continue;
+                e                     = ctl->end[r];
+                const std::uint64_t k = atomicAdd(
+                    reinterpret_cast<unsigned long long*>(&ctl->next[r]),
+                    static_cast<unsigned long long>(res.chunk));
+                if(k < e)
+                {
+                    b   = k;
+                    got = static_cast<std::uint32_t>(e - k < res.chunk ? e - k : res.chunk);
+                }
+                else
+                    exh |= 1u << r;
+            }
+        }
+        res.base      = uniform64(b);
+        res.count     = uniform(got);
+        res.exhausted = uniform(exh);
+        // guided chunking: about 1/256 of what the range still holds
+        const std::uint64_t left = uniform64(e) - (res.base + res.count);
+        const std::uint64_t c    = left / 256;
+        res.chunk = static_cast<std::uint32_t>(c < 64 ? 64 : (c > 512 ? 512 : c));
+    }
+    const std::uint32_t g = res.count < want ? res.count : want;
+    base                  = res.base;
+    res.base += g;
+    res.count -= g;
+    return g;
+}
+
 /// @brief One hybrid pass.  Launched with one wave per workgroup and a grid
 /// sized to the resident wave count; every wave loops claiming batches until
-/// the pass has none left, so each wave exits after finite work.
-template <typename K>
+/// the pass has none left, so each wave exits after finite work.  Waves
+/// beyond the pass's useful work exit at once.
+template <typename K, int P>
 __global__ __launch_bounds__(64) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
@@ -515,16 +731,29 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     HybridCtl* ctl            = h.ctl;
     const std::uint32_t par   = uniform(ctl->parity);
     const std::uint32_t nbq   = uniform(ctl->nbatches);
-    const char*         qin   = h.q[par];
-    char*               qout  = h.q[par ^ 1u];
-    std::uint32_t*      qcnt  = ctl->qcount[par ^ 1u];
-    const std::uint64_t n     = c.n;
     const std::uint32_t lane  = lane_id();
+    const std::uint32_t home  = blockIdx.x % kRanges;
+
+    {
+        // useful waves: queue batches plus input batches left
+        std::uint64_t left = 0;
+        for(std::uint32_t i = 0; i < kRanges; ++i)
+        {
+            const std::uint64_t nx = ctl->next[i], e = ctl->end[i];
+            left += nx < e ? (e - nx + 63) / 64 : 0;
+        }
+        if(std::uint64_t(blockIdx.x) >= nbq + uniform64(left)) return;
+    }
+
+    const char*    qin  = h.q[par];
+    char*          qout = h.q[par ^ 1u];
+    std::uint32_t* qcnt = ctl->qcount[par ^ 1u];
 
     WaveCtx w;
     w.max_steps = c.max_steps;
     std::uint64_t spill_rows = 0, spill_bytes = 0, fill_rows = 0, fill_bytes = 0,
                   admitted = 0, refills = 0, batches = 0, spill_failed = 0;
+    Reservoir res;
 
     Regs<K>       r;
     std::uint32_t pc  = HALT;
@@ -536,16 +765,16 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         // ---- claim a batch (the wave is empty here) ----
         pc              = HALT;
         has             = false;
-        std::uint32_t b = 0;
-        if(lane == 0) b = atomicAdd(&ctl->batch_next, 1u);
+        std::uint32_t b = nbq;
+        if(lane == 0 && nbq) b = atomicAdd(&ctl->batch_next, 1u);
         b = uniform(b);
         if(b < nbq)
         {
             // label of batch b: last label whose prefix is <= b
             std::uint32_t label = 0;
             while(label + 1 < 64 && ctl->bprefix[label + 1] <= b) ++label;
-            label                   = uniform(label);
-            const std::uint32_t cnt = uniform(ctl->qcount[par][label]);
+            label                    = uniform(label);
+            const std::uint32_t cnt  = uniform(ctl->qcount[par][label]);
             const std::uint32_t row0 = (b - uniform(ctl->bprefix[label])) * 64u;
             const std::uint32_t rows = cnt - row0 < 64u ? cnt - row0 : 64u;
             if(lane < rows)
@@ -559,19 +788,18 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         }
         else
         {
-            std::uint64_t k = 0;
-            if(lane == 0) k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next_input), 64ull);
-            k = uniform64(k);
-            if(k >= n) break;
-            const std::uint64_t t = k + lane;
-            if(t < n)
+            std::uint64_t       base = 0;
+            const std::uint32_t got  = take_input(ctl, home, res, 64u, base);
+            if(!got) break;
+            if(lane < got)
             {
+                const std::uint64_t t = base + lane;
                 load_inputs<K>(r, io.in, t);
                 tid = static_cast<std::uint32_t>(t);
                 pc  = 0;
                 has = true;
             }
-            admitted += (n - k < 64 ? n - k : 64);
+            admitted += got;
         }
         ++batches;
 
@@ -579,7 +807,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         bool force = true;
         for(;;)
         {
-            run_to_threshold<K>(r, pc, c.thr, force, w);
+            run_policy<K, P>(r, pc, c.thr, c.live_thr, force, w);
             force = false;
             if(w.over) break;
             if(has && pc == HALT)
@@ -589,57 +817,60 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
             }
             const std::uint64_t live = ballot(has);
             if(!live) break;
-            if(c.refill)
+            if(c.refill && ~live)
             {
                 const std::uint64_t empty = ~live;
-                std::uint64_t       k     = 0;
-                if(lane == 0)
-                    k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next_input),
-                                  static_cast<unsigned long long>(popc(empty)));
-                k = uniform64(k);
-                if(k < n && empty)
+                std::uint64_t       base  = 0;
+                const std::uint32_t got   = take_input(ctl, home, res, popc(empty), base);
+                if(got)
                 {
-                    const std::uint64_t t = k + rank_in(empty);
-                    if(!has && t < n)
+                    const std::uint32_t k = rank_in(empty);
+                    if(!has && k < got)
                     {
+                        const std::uint64_t t = base + k;
                         load_inputs<K>(r, io.in, t);
                         tid = static_cast<std::uint32_t>(t);
                         pc  = 0;
                         has = true;
                     }
-                    admitted += (n - k < popc(empty) ? n - k : popc(empty));
+                    admitted += got;
                     ++refills;
                     force = true;
                     continue;
                 }
             }
-            // spill survivors into the queues of their labels
-            std::uint64_t todo = live;
-            while(todo)
+            if(c.regroup)
             {
-                const int           l     = __builtin_ctzll(todo);
-                const std::uint32_t label = uniform(__builtin_amdgcn_readlane(pc, l));
-                const std::uint64_t m     = ballot(has && pc == label);
-                todo &= ~m;
-                const std::uint32_t cntm = popc(m);
-                std::uint32_t       base = 0;
-                if(lane == 0) base = atomicAdd(&qcnt[label], cntm);
-                base = uniform(base);
-                const std::uint32_t row = base + rank_in(m);
-                if(has && pc == label && row < h.qcap)
+                // spill survivors into the queues of their labels
+                std::uint64_t todo = live;
+                while(todo)
                 {
-                    spill_label<K>(label, r, tid, qout, h.qcap, row);
-                    has = false;
-                }
-                const std::uint32_t ok = base >= h.qcap ? 0u :
+                    const int           l     = __builtin_ctzll(todo);
+                    const std::uint32_t label = uniform(__builtin_amdgcn_readlane(pc, l));
+                    const std::uint64_t m     = ballot(has && pc == label);
+                    todo &= ~m;
+                    const std::uint32_t cntm = popc(m);
+                    std::uint32_t       base = 0;
+                    if(lane == 0) base = atomicAdd(&qcnt[label], cntm);
+                    base = uniform(base);
+                    const std::uint32_t row = base + rank_in(m);
+                    if(has && pc == label && row < h.qcap)
+                    {
+                        spill_label<K>(label, r, tid, qout, h.qcap, row);
+                        has = false;
+                        pc  = HALT;
+                    }
+                    const std::uint32_t ok =
+                        base >= h.qcap ? 0u :
                                          (h.qcap - base < cntm ? std::uint32_t(h.qcap - base) : cntm);
-                spill_rows += ok;
-                spill_bytes += std::uint64_t(ok) * QL::live_bytes(label);
-                spill_failed += cntm - ok;
+                    spill_rows += ok;
+                    spill_bytes += std::uint64_t(ok) * QL::live_bytes(label);
+                    spill_failed += cntm - ok;
+                }
+                if(!ballot(has)) break;
             }
-            if(!ballot(has)) break;
-            // a queue was full: finish the remaining lanes in place
-            run_to_threshold<K>(r, pc, 0, true, w);
+            // no regroup (or a full queue): finish the remaining lanes in place
+            run_policy<K, P>(r, pc, 0, 0, true, w);
             if(has && pc == HALT)
             {
                 store_outputs<K>(r, io.out, tid);
@@ -652,15 +883,15 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     flush_stats<K>(w, c);
     if(c.stats && lane == 0)
     {
-        auto* s = reinterpret_cast<unsigned long long*>(c.stats);
-        atomicAdd(&s[ST_SPILL_ROWS], spill_rows);
-        atomicAdd(&s[ST_SPILL_BYTES], spill_bytes);
-        atomicAdd(&s[ST_FILL_ROWS], fill_rows);
-        atomicAdd(&s[ST_FILL_BYTES], fill_bytes);
-        atomicAdd(&s[ST_ADMITTED], admitted);
-        atomicAdd(&s[ST_REFILLS], refills);
-        atomicAdd(&s[ST_BATCHES], batches);
-        atomicAdd(&s[ST_SPILL_FAILED], spill_failed);
+        auto* st = reinterpret_cast<unsigned long long*>(c.stats);
+        atomicAdd(&st[ST_SPILL_ROWS], spill_rows);
+        atomicAdd(&st[ST_SPILL_BYTES], spill_bytes);
+        atomicAdd(&st[ST_FILL_ROWS], fill_rows);
+        atomicAdd(&st[ST_FILL_BYTES], fill_bytes);
+        atomicAdd(&st[ST_ADMITTED], admitted);
+        atomicAdd(&st[ST_REFILLS], refills);
+        atomicAdd(&st[ST_BATCHES], batches);
+        atomicAdd(&st[ST_SPILL_FAILED], spill_failed);
     }
     raise_errors(w.err, w.over, c);
 }
@@ -676,15 +907,19 @@ hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
     if(threadIdx.x != 0) return;
     if(first)
     {
-        ctl->next_input = 0;
+        for(std::uint32_t r = 0; r < kRanges; ++r)
+        {
+            ctl->next[r] = n * r / kRanges;
+            ctl->end[r]  = n * (r + 1) / kRanges;
+        }
         ctl->batch_next = 0;
         ctl->nbatches   = 0;
         ctl->parity     = 0;
         ctl->live       = 0;
         ctl->passes     = 0;
         ctl->done       = n == 0;
-        for(int s = 0; s < 2; ++s)
-            for(int l = 0; l < 64; ++l) ctl->qcount[s][l] = 0;
+        for(int side = 0; side < 2; ++side)
+            for(int l = 0; l < 64; ++l) ctl->qcount[side][l] = 0;
         for(int l = 0; l < 65; ++l) ctl->bprefix[l] = 0;
         return;
     }
@@ -701,11 +936,17 @@ hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
     }
     ctl->bprefix[64] = pre;
     for(std::uint32_t l = 0; l < 64; ++l) ctl->qcount[out ^ 1u][l] = 0;
+    bool input_left = false;
+    for(std::uint32_t r = 0; r < kRanges; ++r)
+    {
+        if(ctl->next[r] > ctl->end[r]) ctl->next[r] = ctl->end[r];
+        input_left |= ctl->next[r] < ctl->end[r];
+    }
     ctl->parity     = out;
     ctl->nbatches   = pre;
     ctl->batch_next = 0;
     ctl->live       = live;
     ctl->passes += 1;
-    ctl->done = live == 0 && ctl->next_input >= n;
+    ctl->done = live == 0 && !input_left;
 }
 } // namespace loom::hipvm

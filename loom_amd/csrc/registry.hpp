@@ -22,11 +22,12 @@ namespace loom::hipvm
 struct KernelEntry
 {
     loom_kernel_info info{};
-    hipError_t (*compact)(const IoArgs&, const CtlArgs&, hipStream_t) = nullptr;
-    hipError_t (*hybrid_pass)(const IoArgs&, const CtlArgs&, const HybridArgs&,
-                              unsigned grid, hipStream_t)           = nullptr;
+    /// launchers per reconvergence policy (POLICY_SUCCESSOR_MAJOR, POLICY_TOPO)
+    hipError_t (*compact[2])(const IoArgs&, const CtlArgs&, hipStream_t) = {};
+    hipError_t (*hybrid_pass[2])(const IoArgs&, const CtlArgs&, const HybridArgs&,
+                                 unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
-    int (*hybrid_occupancy)()                                       = nullptr;
+    int (*hybrid_occupancy[2])()                                       = {};
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
     int (*kernel_attrs)(int which, int* vgprs, int* sgprs, int* scratch) = nullptr;
 };
@@ -34,32 +35,32 @@ struct KernelEntry
 /// @brief Add @p e to the process-wide registry; returns true.
 bool register_kernel(const KernelEntry& e);
 
-template <typename K>
+template <typename K, int P>
 hipError_t
 launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
 {
     if(c.n == 0) return hipSuccess;
     const std::uint64_t blocks = (c.n + 255) / 256;
-    hipLaunchKernelGGL(compact_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+    hipLaunchKernelGGL((compact_kernel<K, P>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                        io, c);
     return hipGetLastError();
 }
 
-template <typename K>
+template <typename K, int P>
 hipError_t
 launch_hybrid_pass(const IoArgs& io, const CtlArgs& c, const HybridArgs& h, unsigned grid,
                    hipStream_t s)
 {
-    hipLaunchKernelGGL(hybrid_pass_kernel<K>, dim3(grid), dim3(64), 0, s, io, c, h);
+    hipLaunchKernelGGL((hybrid_pass_kernel<K, P>), dim3(grid), dim3(64), 0, s, io, c, h);
     return hipGetLastError();
 }
 
-template <typename K>
+template <typename K, int P>
 int
 hybrid_occupancy()
 {
     int nblk = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K>, 64, 0) !=
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K, P>, 64, 0) !=
        hipSuccess)
         return 1;
     return nblk > 0 ? nblk : 1;
@@ -71,8 +72,8 @@ kernel_attrs(int which, int* vgprs, int* sgprs, int* scratch)
 {
     hipFuncAttributes a{};
     const hipError_t  e =
-        which == 0 ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(compact_kernel<K>)) :
-                     hipFuncGetAttributes(&a, reinterpret_cast<const void*>(hybrid_pass_kernel<K>));
+        which == 0 ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(compact_kernel<K, POLICY_TOPO>)) :
+                     hipFuncGetAttributes(&a, reinterpret_cast<const void*>(hybrid_pass_kernel<K, POLICY_TOPO>));
     if(e != hipSuccess) return static_cast<int>(e);
     if(vgprs) *vgprs = a.numRegs;
     if(sgprs) *sgprs = 0;
@@ -104,10 +105,14 @@ make_entry(const char* name)
     {
         e.info.block_ops[b]        = T.block[b].ops;
         e.info.block_live_bytes[b] = QueueLayout<K>::live_bytes(b);
+        e.info.block_flops[b]      = T.block[b].flops;
     }
-    e.compact          = &launch_compact<K>;
-    e.hybrid_pass      = &launch_hybrid_pass<K>;
-    e.hybrid_occupancy = &hybrid_occupancy<K>;
+    e.compact[0]          = &launch_compact<K, POLICY_SUCCESSOR_MAJOR>;
+    e.compact[1]          = &launch_compact<K, POLICY_TOPO>;
+    e.hybrid_pass[0]      = &launch_hybrid_pass<K, POLICY_SUCCESSOR_MAJOR>;
+    e.hybrid_pass[1]      = &launch_hybrid_pass<K, POLICY_TOPO>;
+    e.hybrid_occupancy[0] = &hybrid_occupancy<K, POLICY_SUCCESSOR_MAJOR>;
+    e.hybrid_occupancy[1] = &hybrid_occupancy<K, POLICY_TOPO>;
     e.kernel_attrs     = &kernel_attrs<K>;
     return e;
 }

@@ -146,10 +146,10 @@ finish_stats(loom_run_stats* st, const loom_params* p)
 }
 
 int
-run_compact(const KernelEntry& e, const IoArgs& io, CtlArgs c, loom_run_stats* st)
+run_compact(const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy, loom_run_stats* st)
 {
     LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    LOOM_HIP(e.compact(io, c, g_dev.stream));
+    LOOM_HIP(e.compact[policy](io, c, g_dev.stream));
     LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
     LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
     float ms = 0;
@@ -170,7 +170,8 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
            loom_run_stats* st)
 {
     // persistent grid: resident 64-lane workgroups
-    const int occ = e.hybrid_occupancy();
+    const int policy = p ? static_cast<int>(p->reconv) : 1;
+    const int occ    = e.hybrid_occupancy[policy]();
     int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
     wpc           = std::max(1, std::min(wpc, occ));
     unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
@@ -228,7 +229,7 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
             hipEvent_t a = next_ev(), b = next_ev();
             if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
             LOOM_HIP(hipEventRecord(a, g_dev.stream));
-            LOOM_HIP(e.hybrid_pass(io, c, h, grid, g_dev.stream));
+            LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
             LOOM_HIP(hipEventRecord(b, g_dev.stream));
             evs.emplace_back(a, b);
             hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream, ws.ctl,
@@ -427,14 +428,18 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
         if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
     for(int k = 0; n && k < e.info.n_outputs; ++k)
         if(!outputs[k]) return fail(LOOM_E_ARG, "output column %d is null", k);
-    if(params && params->threshold > 63) return fail(LOOM_E_ARG, "threshold must be < 64");
+    if(params && (params->threshold > 63 || params->live_threshold > 63))
+        return fail(LOOM_E_ARG, "thresholds must be < 64");
+    if(params && params->reconv > 1) return fail(LOOM_E_ARG, "unknown reconvergence policy");
     if(stats) *stats = loom_run_stats{};
     const IoArgs io = n ? make_io(e, inputs, outputs) : IoArgs{};
     CtlArgs      c;
     c.n         = n;
     c.max_steps = params && params->max_steps ? params->max_steps : (1ull << 40);
     c.thr       = params ? params->threshold : 0;
+    c.live_thr  = params ? params->live_threshold : 0;
     c.refill    = params ? params->refill : 1;
+    c.regroup   = params ? params->regroup : 1;
     c.err       = g_dev.err;
     c.stats     = params && params->collect_stats ? g_dev.stats : nullptr;
     LOOM_HIP(hipMemsetAsync(g_dev.err, 0, sizeof(std::uint32_t), g_dev.stream));
@@ -442,7 +447,7 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
         LOOM_HIP(hipMemsetAsync(g_dev.stats, 0, sizeof(std::uint64_t) * kNumStats, g_dev.stream));
     int r = 0;
     if(mode == LOOM_MODE_COMPACT)
-        r = run_compact(e, io, c, stats);
+        r = run_compact(e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
     else if(mode == LOOM_MODE_HYBRID)
         r = run_hybrid(kernel, e, io, c, params, stats);
     else

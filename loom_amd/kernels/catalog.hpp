@@ -6,6 +6,7 @@
 #pragma once
 
 #include "programs.hpp"
+#include "smallpt9.hpp"
 
 #define LOOM_AMD_CATALOG_CORE(X)                                      \
     X("fib_u64", loom_amd::programs::fib<std::uint64_t>)              \
@@ -17,3 +18,7 @@
     X("int_mix_u32", loom_amd::programs::int_mix<std::uint32_t>)      \
     X("fp_mix_f32", loom_amd::programs::fp_mix<float>)                \
     X("fp_mix_f64", loom_amd::programs::fp_mix<double>)
+
+#define LOOM_AMD_CATALOG_PT(X) X("smallpt9_f32", loom_amd::smallpt::smallpt9<float>)
+
+#define LOOM_AMD_CATALOG(X) LOOM_AMD_CATALOG_CORE(X) LOOM_AMD_CATALOG_PT(X)

@@ -42,3 +42,29 @@ def fib_inputs(n: int = 1 << 20, seed: int = SEED) -> np.ndarray:
 def collatz_inputs(n: int = 1 << 24, seed: int = SEED + 1) -> np.ndarray:
     """Collatz config: start values uniform in [1, 2^32)."""
     return uniform_int(seed, n, 1, (1 << 32) - 1)
+
+
+def pt_job(w: int, h: int, spp: int) -> int:
+    """The `job` input word of the smallpt9 kernel: w<<20 | h<<8 | spp."""
+    assert 0 < w < 4096 and 0 < h < 4096 and 0 < spp < 256
+    return (w << 20) | (h << 8) | spp
+
+
+def pt_inputs(w: int = 1024, h: int = 768, spp: int = 64):
+    """Path-tracing config: one thread per sample, sample index sid and the
+    frame word (identical for every thread of the frame)."""
+    n = w * h * spp
+    return [np.arange(n, dtype=np.uint32), np.full(n, pt_job(w, h, spp), dtype=np.uint32)]
+
+
+def pt_image(lx, ly, lz, w: int, h: int, spp: int) -> np.ndarray:
+    """Pixel radiance: the mean of each pixel's spp samples, summed in sample
+    order in float32 (rows bottom to top as generated)."""
+    out = np.zeros((h * w, 3), np.float32)
+    for c, col in enumerate((lx, ly, lz)):
+        acc = np.zeros(h * w, np.float32)
+        s = np.asarray(col, np.float32).reshape(h * w, spp)
+        for k in range(spp):
+            acc = (acc + s[:, k]).astype(np.float32)
+        out[:, c] = acc / np.float32(spp)
+    return out.reshape(h, w, 3)

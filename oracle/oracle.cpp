@@ -4,6 +4,7 @@
 /// restatement, for the parity tests and the CPU baseline timing.  Never
 /// linked into or called by the product library.
 #include "bse.hpp"
+#include "smallpt9_direct.hpp"
 
 #include "../loom_amd/kernels/catalog.hpp"
 
@@ -24,7 +25,7 @@ struct Entry
 };
 
 #define LOOM_ORACLE_ENTRY(NAME, ...) Entry{NAME, &build_program<__VA_ARGS__>},
-const Entry g_entries[] = {LOOM_AMD_CATALOG_CORE(LOOM_ORACLE_ENTRY)};
+const Entry g_entries[] = {LOOM_AMD_CATALOG(LOOM_ORACLE_ENTRY)};
 #undef LOOM_ORACLE_ENTRY
 
 constexpr int g_count = sizeof(g_entries) / sizeof(g_entries[0]);
@@ -169,6 +170,35 @@ oracle_parse_i32(const char* s, int32_t* v)
     auto r = lit::parse<std::int32_t>(s);
     if(!r) return 1;
     *v = *r;
+    return 0;
+}
+
+/// The direct C++ path tracer on samples [begin, begin+count) of host columns.
+int
+oracle_smallpt9_direct(const uint32_t* sid, const uint32_t* job, float* lx, float* ly, float* lz,
+                       uint64_t begin, uint64_t count, int threads)
+{
+    static const direct::Scene scene;
+    auto work = [&](std::uint64_t b, std::uint64_t e)
+    {
+        for(std::uint64_t i = b; i < e; ++i) direct::sample(scene, sid[i], job[i], lx[i], ly[i], lz[i]);
+    };
+    threads = std::max(1, threads);
+    if(threads == 1 || count < 1024)
+        work(begin, begin + count);
+    else
+    {
+        std::vector<std::thread> pool;
+        const std::uint64_t      chunk = (count + threads - 1) / threads;
+        for(int t = 0; t < threads; ++t)
+        {
+            const std::uint64_t b = begin + chunk * t;
+            const std::uint64_t e = std::min(begin + count, b + chunk);
+            if(b >= e) break;
+            pool.emplace_back(work, b, e);
+        }
+        for(auto& th: pool) th.join();
+    }
     return 0;
 }
 

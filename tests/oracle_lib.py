@@ -34,6 +34,7 @@ def lib():
                 fn = getattr(L, f"oracle_{f}_{nm}")
                 fn.argtypes = [t]
                 fn.restype = t
+        L.oracle_smallpt9_direct.argtypes = [C.c_void_p] * 5 + [C.c_uint64, C.c_uint64, C.c_int]
         L.oracle_parse_u64.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
         L.oracle_parse_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
         _lib = L
@@ -69,3 +70,14 @@ def parse(kind, text):
     v = {"f32": C.c_float, "f64": C.c_double, "u64": C.c_uint64, "i32": C.c_int32}[kind]()
     rc = getattr(lib(), f"oracle_parse_{kind}")(text.encode(), C.byref(v))
     return None if rc else v.value
+
+
+def smallpt9_direct(sid, job, threads=8):
+    """The direct C++ path tracer (not the IR) on host columns."""
+    sid = np.ascontiguousarray(sid, np.uint32)
+    job = np.ascontiguousarray(job, np.uint32)
+    n = len(sid)
+    outs = [np.zeros(n, np.float32) for _ in range(3)]
+    lib().oracle_smallpt9_direct(sid.ctypes.data, job.ctypes.data, outs[0].ctypes.data,
+                                 outs[1].ctypes.data, outs[2].ctypes.data, 0, n, threads)
+    return outs

@@ -14,16 +14,25 @@ pytestmark = pytest.mark.gpu
 
 MODES = [
     ("compact", abi.MODE_COMPACT, {}),
-    ("hybrid_t0", abi.MODE_HYBRID, dict(threshold=0)),
-    ("hybrid_t32", abi.MODE_HYBRID, dict(threshold=32)),
-    ("hybrid_t56_norefill", abi.MODE_HYBRID, dict(threshold=56, refill=False)),
-    ("hybrid_t48_smallq", abi.MODE_HYBRID, dict(threshold=48, queue_capacity=64, waves_per_cu=1)),
+    ("compact_topo", abi.MODE_COMPACT, dict(reconv=abi.RECONV_TOPO)),
+    ("hybrid_t0", abi.MODE_HYBRID, dict(threshold=0, reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("hybrid_topo_live48", abi.MODE_HYBRID, dict(live_threshold=48)),
+    ("hybrid_sm_live48", abi.MODE_HYBRID, dict(live_threshold=48,
+                                               reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("hybrid_live60_noregroup", abi.MODE_HYBRID, dict(live_threshold=60, regroup=False)),
+    ("hybrid_t56_norefill", abi.MODE_HYBRID, dict(threshold=56, refill=False,
+                                                  reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("hybrid_t48_smallq", abi.MODE_HYBRID, dict(threshold=48, live_threshold=40,
+                                                queue_capacity=64, waves_per_cu=1,
+                                                reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("hybrid_topo_smallq", abi.MODE_HYBRID, dict(live_threshold=40, queue_capacity=64,
+                                                 waves_per_cu=1)),
 ]
 STEP_BUDGET = 1 << 24  # every kernel here finishes far below it
 
 
 def gpu_run(name, mode, params, inputs):
-    p = loom_amd.make_params(max_steps=STEP_BUDGET, **params)
+    p = loom_amd.make_params(max_steps=STEP_BUDGET, mode=mode, **params)
     print(f"[gpu] {name} mode={mode} params={params} n={len(inputs[0])}", flush=True)
     r = loom_amd.run(name, mode, inputs, params=p)
     assert r.errors == 0
@@ -59,6 +68,8 @@ CASES = {
                              np.random.default_rng(19).uniform(-50, 50, n).astype(np.float32)],
     "fp_mix_f64": lambda n: [np.random.default_rng(20).uniform(-900, 900, n),
                              np.random.default_rng(21).uniform(-50, 50, n)],
+    # path tracer: the first n samples of a 40x30 frame at 8 spp
+    "smallpt9_f32": lambda n: [a[:n] for a in workloads.pt_inputs(40, 30, 8)],
 }
 
 
@@ -116,3 +127,18 @@ def test_collatz_full_config_forms_agree(gpu):
     assert [int(a[i]) for i in idx] == [collatz_ref(int(x[i])) for i in idx]
     (w,), _ = O.run("collatz_u64", [x[:1 << 16]])
     assert np.array_equal(a[:1 << 16], w)
+
+
+@pytest.mark.parametrize("mname,mode,params", MODES, ids=[m[0] for m in MODES])
+def test_smallpt9_frame_vs_oracle(gpu, mname, mode, params):
+    """A whole 64x48x16spp frame: per-sample radiance bit-exact vs the oracle,
+    image finite and non-negative."""
+    w, h, spp = 64, 48, 16
+    ins = workloads.pt_inputs(w, h, spp)
+    want, err = O.run("smallpt9_f32", ins)
+    assert err == 0
+    got = gpu_run("smallpt9_f32", mode, params, ins).outputs
+    for g, o in zip(got, want):
+        assert same(g, o), mname
+    img = workloads.pt_image(*got, w, h, spp)
+    assert np.isfinite(img).all() and (img >= 0).all()
