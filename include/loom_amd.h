@@ -158,6 +158,37 @@ int loom_run_host(int kernel, int mode, const void* const* inputs,
                   void* const* outputs, uint64_t n, const loom_params* params,
                   loom_run_stats* stats);
 
+/* ---- batched launches ------------------------------------------------------
+ * Replaces the multi-kernel Shop (src/vm/mp.hpp:316-385: bind_inputs, the
+ * workers' round-robin over per-kernel streams, finalize): the jobs of one
+ * call run in ONE persistent launch; every job's kernel must belong to the
+ * compiled batch set.  Jobs read/write device columns in [begin, end). */
+#define LOOM_MAX_JOBS 64
+
+typedef struct loom_job
+{
+    int32_t     kernel; /* global kernel index, a member of the batch set */
+    int32_t     pad;
+    uint64_t    begin, end;
+    const void* inputs[LOOM_MAX_IO];
+    void*       outputs[LOOM_MAX_IO];
+} loom_job;
+
+int loom_batch_set_count(void);
+int loom_batch_set_find(const char* name);
+/* global kernel indices of the members; returns their count (< 0: error) */
+int loom_batch_set_members(int set, int* kernels, int max);
+int loom_batch_run(int set, const loom_job* jobs, int njobs,
+                   const loom_params* params, loom_run_stats* stats);
+
+/* A batch captured once into a HIP graph and replayed (no counterpart in
+ * the CPU reference).  The jobs are fixed at capture time. */
+typedef struct loom_graph loom_graph;
+int loom_batch_graph_create(int set, const loom_job* jobs, int njobs,
+                            const loom_params* params, loom_graph** graph);
+int loom_graph_launch(loom_graph* graph, loom_run_stats* stats);
+int loom_graph_destroy(loom_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -245,3 +245,87 @@ class hybrid_function(compact_function):
         super().__init__(k, threshold=threshold, live_threshold=live_threshold, refill=refill,
                          regroup=regroup, waves_per_cu=waves_per_cu,
                          queue_capacity=queue_capacity, **params)
+
+
+# ---- batched launches (reference mp::Shop, src/vm/mp.hpp:316-385) ------------------
+def _result(st, outputs=None):
+    return RunResult(outputs=outputs, ms_total=st.ms_total, ms_main=st.ms_main,
+                     main_launches=st.main_launches, passes=st.passes, errors=st.errors,
+                     grid_waves=st.grid_waves, counters=list(st.counters))
+
+
+class Batch:
+    """Jobs of several kernels run by ONE persistent launch of a compiled
+    batch set.  `add(kernel, inputs, outputs, begin, end)` takes DeviceArray
+    columns; `run()` executes every job to completion; `graph()` captures the
+    batch into a HIP graph for replay."""
+
+    def __init__(self, set_name: str):
+        init()
+        lib = abi.load()
+        self.set = lib.loom_batch_set_find(set_name.encode())
+        if self.set < 0:
+            raise KeyError(f"no batch set {set_name!r}")
+        buf = (C.c_int * 64)()
+        n = lib.loom_batch_set_members(self.set, buf, 64)
+        self.members = [kernel(buf[i]) for i in range(n)]
+        self.jobs = []
+        self._keep = []
+
+    def add(self, k, inputs, outputs, begin=0, end=None):
+        k = kernel(k)
+        if k.hash not in {m.hash for m in self.members}:
+            raise ValueError(f"{k.name} is not a member of this batch set")
+        if len(self.jobs) >= abi.MAX_JOBS:
+            raise ValueError("a batch holds at most 64 jobs")
+        j = abi.Job()
+        j.kernel = k.index
+        j.begin = int(begin)
+        j.end = int(inputs[0].n if end is None else end)
+        for i, a in enumerate(inputs):
+            j.inputs[i] = a.ptr
+        for i, a in enumerate(outputs):
+            j.outputs[i] = a.ptr
+        self.jobs.append(j)
+        self._keep.append((inputs, outputs))
+        return len(self.jobs) - 1
+
+    def _table(self):
+        return (abi.Job * len(self.jobs))(*self.jobs)
+
+    def run(self, params=None) -> RunResult:
+        params = params or make_params(mode=abi.MODE_HYBRID, live_threshold=48)
+        st = abi.RunStats()
+        abi.check(abi.load().loom_batch_run(self.set, self._table(), len(self.jobs),
+                                            C.byref(params), C.byref(st)), "loom_batch_run")
+        return _result(st)
+
+    def graph(self, params=None) -> "Graph":
+        params = params or make_params(mode=abi.MODE_HYBRID, live_threshold=48)
+        g = C.c_void_p()
+        abi.check(abi.load().loom_batch_graph_create(self.set, self._table(), len(self.jobs),
+                                                     C.byref(params), C.byref(g)),
+                  "loom_batch_graph_create")
+        return Graph(g, self._keep)
+
+
+class Graph:
+    def __init__(self, handle, keep):
+        self.handle = handle
+        self._keep = keep
+
+    def launch(self) -> RunResult:
+        st = abi.RunStats()
+        abi.check(abi.load().loom_graph_launch(self.handle, C.byref(st)), "loom_graph_launch")
+        return _result(st)
+
+    def destroy(self):
+        if self.handle:
+            abi.load().loom_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

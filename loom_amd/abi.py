@@ -76,6 +76,20 @@ class RunStats(C.Structure):
     ]
 
 
+MAX_JOBS = 64
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("pad", C.c_int32),
+        ("begin", C.c_uint64),
+        ("end", C.c_uint64),
+        ("inputs", C.c_void_p * MAX_IO),
+        ("outputs", C.c_void_p * MAX_IO),
+    ]
+
+
 # every symbol declared in include/loom_amd.h, with its signature
 SIGNATURES = {
     "loom_init": (C.c_int, [C.c_int]),
@@ -97,6 +111,15 @@ SIGNATURES = {
     "loom_run_host": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p),
                                 C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
                                 C.POINTER(RunStats)]),
+    "loom_batch_set_count": (C.c_int, []),
+    "loom_batch_set_find": (C.c_int, [C.c_char_p]),
+    "loom_batch_set_members": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int]),
+    "loom_batch_run": (C.c_int, [C.c_int, C.POINTER(Job), C.c_int, C.POINTER(Params),
+                                 C.POINTER(RunStats)]),
+    "loom_batch_graph_create": (C.c_int, [C.c_int, C.POINTER(Job), C.c_int, C.POINTER(Params),
+                                          C.POINTER(C.c_void_p)]),
+    "loom_graph_launch": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "loom_graph_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

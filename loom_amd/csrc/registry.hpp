@@ -8,7 +8,7 @@
 /// index or name.
 #pragma once
 
-#include "device/sched.hpp"
+#include "device/batch.hpp"
 
 #include <loom_amd.h>
 
@@ -34,6 +34,51 @@ struct KernelEntry
 
 /// @brief Add @p e to the process-wide registry; returns true.
 bool register_kernel(const KernelEntry& e);
+
+/// @brief A compiled batch set: kernels that one persistent launch can mix.
+struct BatchEntry
+{
+    char          name[64]{};
+    std::uint32_t size = 0;
+    std::uint64_t hash[kMaxJobs]{}; ///< structural hashes of the members
+    hipError_t (*launch)(const CtlArgs&, const BatchArgs&, unsigned grid, hipStream_t) = nullptr;
+    int (*occupancy)() = nullptr;
+};
+
+bool register_batch(const BatchEntry& e);
+
+template <typename... Ks>
+hipError_t
+launch_batch(const CtlArgs& c, const BatchArgs& a, unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_kernel<Ks...>), dim3(grid), dim3(64), 0, s, c, a);
+    return hipGetLastError();
+}
+
+template <typename... Ks>
+int
+batch_occupancy()
+{
+    int nblk = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, batch_kernel<Ks...>, 64, 0) !=
+       hipSuccess)
+        return 1;
+    return nblk > 0 ? nblk : 1;
+}
+
+template <typename... Ks>
+BatchEntry
+make_batch(const char* name)
+{
+    BatchEntry e;
+    std::strncpy(e.name, name, sizeof(e.name) - 1);
+    e.size        = sizeof...(Ks);
+    std::size_t i = 0;
+    ((e.hash[i++] = ir::table_of<Ks>.hash), ...);
+    e.launch    = &launch_batch<Ks...>;
+    e.occupancy = &batch_occupancy<Ks...>;
+    return e;
+}
 
 template <typename K, int P>
 hipError_t
@@ -126,3 +171,9 @@ make_entry(const char* name)
 
 /// @brief Lower kernel type `...` and register it under @p NAME.
 #define LOOM_HIP_REGISTER(NAME, ...) LOOM_HIP_REGISTER_I(__COUNTER__, NAME, __VA_ARGS__)
+
+/// @brief Compile one persistent kernel mixing the kernel types `...` and
+/// register it as batch set @p NAME.
+#define LOOM_HIP_REGISTER_BATCH(NAME, ...)                                   \
+    static const bool LOOM_REG_CAT(loom_batch_registered_, __COUNTER__) =    \
+        ::loom::hipvm::register_batch(::loom::hipvm::make_batch<__VA_ARGS__>(NAME))

@@ -29,6 +29,13 @@ registry()
     return r;
 }
 
+std::vector<BatchEntry>&
+batch_registry()
+{
+    static std::vector<BatchEntry> r;
+    return r;
+}
+
 thread_local std::string g_error;
 
 int
@@ -64,6 +71,8 @@ struct Device
     HybridCtl*     host_ctl = nullptr;   // pinned mirror of a control block
     hipEvent_t     ev[4]{};
     std::vector<hipEvent_t> pass_ev;
+    JobDesc*       jobs = nullptr; // device job table of direct batch runs
+    BatchCtl*      bctl = nullptr;
 };
 
 Device g_dev;
@@ -265,12 +274,92 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
     }
     return 0;
 }
+__global__ void
+zero_counters_kernel(std::uint32_t* err, std::uint64_t* stats, std::uint32_t nstats)
+{
+    const std::uint32_t i = threadIdx.x;
+    if(i == 0) *err = 0;
+    if(stats && i < nstats) stats[i] = 0;
+}
+
+int
+batch_jobs(int set, const loom_job* jobs, int njobs, std::vector<JobDesc>& out)
+{
+    if(set < 0 || set >= static_cast<int>(batch_registry().size()))
+        return fail(LOOM_E_KERNEL, "unknown batch set %d", set);
+    if(njobs < 1 || njobs > static_cast<int>(kMaxJobs) || !jobs)
+        return fail(LOOM_E_ARG, "a batch holds 1..%u jobs", kMaxJobs);
+    const BatchEntry& b = batch_registry()[set];
+    out.assign(njobs, JobDesc{});
+    for(int i = 0; i < njobs; ++i)
+    {
+        const loom_job& j = jobs[i];
+        if(int r = check_kernel(j.kernel)) return r;
+        const KernelEntry& e = registry()[j.kernel];
+        std::uint32_t      m = b.size;
+        for(std::uint32_t k = 0; k < b.size; ++k)
+            if(b.hash[k] == e.info.hash) m = k;
+        if(m == b.size)
+            return fail(LOOM_E_KERNEL, "kernel %s is not in batch set %s", e.info.name, b.name);
+        if(j.end < j.begin || j.end > 0xFFFFFFFFull)
+            return fail(LOOM_E_ARG, "job %d: bad thread range", i);
+        out[i].member = m;
+        out[i].begin  = j.begin;
+        out[i].end    = j.end;
+        for(int k = 0; k < e.info.n_inputs; ++k)
+        {
+            if(!j.inputs[k] && j.end > j.begin) return fail(LOOM_E_ARG, "job %d input %d null", i, k);
+            out[i].in[k] = j.inputs[k];
+        }
+        for(int k = 0; k < e.info.n_outputs; ++k)
+        {
+            if(!j.outputs[k] && j.end > j.begin) return fail(LOOM_E_ARG, "job %d output %d null", i, k);
+            out[i].out[k] = j.outputs[k];
+        }
+    }
+    return 0;
+}
+
+CtlArgs
+make_ctl(const loom_params* p, std::uint64_t n)
+{
+    CtlArgs c;
+    c.n         = n;
+    c.max_steps = p && p->max_steps ? p->max_steps : (1ull << 40);
+    c.thr       = p ? p->threshold : 0;
+    c.live_thr  = p ? p->live_threshold : 48;
+    c.refill    = p ? p->refill : 1;
+    c.regroup   = p ? p->regroup : 0;
+    c.err       = g_dev.err;
+    c.stats     = p && p->collect_stats ? g_dev.stats : nullptr;
+    return c;
+}
+
+unsigned
+batch_grid(const BatchEntry& b, const loom_params* p, std::uint64_t threads)
+{
+    const int occ = b.occupancy();
+    int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
+    wpc           = std::max(1, std::min(wpc, occ));
+    unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
+    const std::uint64_t need = (threads + 63) / 64;
+    if(need < grid) grid = static_cast<unsigned>(std::max<std::uint64_t>(1, need));
+    return grid;
+}
+
 } // namespace
 
 bool
 register_kernel(const KernelEntry& e)
 {
     registry().push_back(e);
+    return true;
+}
+
+bool
+register_batch(const BatchEntry& e)
+{
+    batch_registry().push_back(e);
     return true;
 }
 } // namespace loom::hipvm
@@ -495,6 +584,182 @@ loom_run_host(int kernel, int mode, const void* const* inputs, void* const* outp
         LOOM_HIP(hipMemcpyAsync(outputs[k], dout[k], n * type_size(e.info.output_type[k]),
                                 hipMemcpyDeviceToHost, g_dev.stream));
     LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_batch_set_count(void)
+{
+    return static_cast<int>(batch_registry().size());
+}
+
+int
+loom_batch_set_find(const char* name)
+{
+    if(!name) return -1;
+    for(std::size_t i = 0; i < batch_registry().size(); ++i)
+        if(std::strcmp(batch_registry()[i].name, name) == 0) return static_cast<int>(i);
+    fail(LOOM_E_KERNEL, "no batch set named %s", name);
+    return -1;
+}
+
+int
+loom_batch_set_members(int set, int* kernels, int max)
+{
+    if(set < 0 || set >= static_cast<int>(batch_registry().size()))
+        return -fail(LOOM_E_KERNEL, "unknown batch set %d", set);
+    const BatchEntry& b = batch_registry()[set];
+    int               n = 0;
+    for(std::uint32_t k = 0; k < b.size; ++k)
+        for(std::size_t g = 0; g < registry().size(); ++g)
+            if(registry()[g].info.hash == b.hash[k])
+            {
+                if(kernels && n < max) kernels[n] = static_cast<int>(g);
+                ++n;
+                break;
+            }
+    return n;
+}
+
+int
+loom_batch_run(int set, const loom_job* jobs, int njobs, const loom_params* params,
+               loom_run_stats* stats)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    std::vector<JobDesc> jd;
+    if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
+    if(stats) *stats = loom_run_stats{};
+    if(!g_dev.jobs) LOOM_HIP(hipMalloc(&g_dev.jobs, sizeof(JobDesc) * kMaxJobs));
+    if(!g_dev.bctl) LOOM_HIP(hipMalloc(&g_dev.bctl, sizeof(BatchCtl)));
+    LOOM_HIP(hipMemcpyAsync(g_dev.jobs, jd.data(), sizeof(JobDesc) * jd.size(),
+                            hipMemcpyHostToDevice, g_dev.stream));
+    std::uint64_t threads = 0;
+    for(const auto& j: jd) threads += j.end - j.begin;
+    const BatchEntry& b = batch_registry()[set];
+    const CtlArgs     c = make_ctl(params, threads);
+    BatchArgs         a;
+    a.jobs  = g_dev.jobs;
+    a.ctl   = g_dev.bctl;
+    a.njobs = static_cast<std::uint32_t>(njobs);
+    const unsigned grid = batch_grid(b, params, threads);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
+                       g_dev.stats, static_cast<std::uint32_t>(kNumStats));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
+    LOOM_HIP(hipGetLastError());
+    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
+    LOOM_HIP(b.launch(c, a, grid, g_dev.stream));
+    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    float ms = 0;
+    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
+    if(stats)
+    {
+        stats->ms_total      = ms;
+        stats->ms_main       = ms;
+        stats->main_launches = 1;
+        stats->passes        = 1;
+        stats->grid_waves    = grid;
+    }
+    return finish_stats(stats, params);
+}
+
+} // extern "C"
+
+struct loom_graph
+{
+    hipGraph_t     graph{};
+    hipGraphExec_t exec{};
+    JobDesc*       jobs = nullptr;
+    BatchCtl*      ctl  = nullptr;
+    loom_params    params{};
+    bool           has_params = false;
+    unsigned       grid       = 0;
+};
+
+extern "C" {
+
+int
+loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_params* params,
+                        loom_graph** graph)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(!graph) return fail(LOOM_E_ARG, "null graph");
+    std::vector<JobDesc> jd;
+    if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
+    auto g = new loom_graph;
+    if(params)
+    {
+        g->params     = *params;
+        g->has_params = true;
+    }
+    LOOM_HIP(hipMalloc(&g->jobs, sizeof(JobDesc) * kMaxJobs));
+    LOOM_HIP(hipMalloc(&g->ctl, sizeof(BatchCtl)));
+    LOOM_HIP(hipMemcpyAsync(g->jobs, jd.data(), sizeof(JobDesc) * jd.size(),
+                            hipMemcpyHostToDevice, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    std::uint64_t threads = 0;
+    for(const auto& j: jd) threads += j.end - j.begin;
+    const BatchEntry& b = batch_registry()[set];
+    const CtlArgs     c = make_ctl(params, threads);
+    BatchArgs         a;
+    a.jobs  = g->jobs;
+    a.ctl   = g->ctl;
+    a.njobs = static_cast<std::uint32_t>(njobs);
+    g->grid = batch_grid(b, params, threads);
+    // counters are cleared by a kernel node (no memset nodes in the graph)
+    LOOM_HIP(hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
+                       g_dev.stats, static_cast<std::uint32_t>(kNumStats));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
+    const hipError_t le = b.launch(c, a, g->grid, g_dev.stream);
+    const hipError_t ce = hipStreamEndCapture(g_dev.stream, &g->graph);
+    if(le != hipSuccess || ce != hipSuccess)
+    {
+        delete g;
+        return fail(LOOM_E_RUNTIME, "graph capture failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    LOOM_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *graph = g;
+    return 0;
+}
+
+int
+loom_graph_launch(loom_graph* g, loom_run_stats* stats)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(!g) return fail(LOOM_E_ARG, "null graph");
+    if(stats) *stats = loom_run_stats{};
+    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
+    LOOM_HIP(hipGraphLaunch(g->exec, g_dev.stream));
+    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    float ms = 0;
+    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
+    if(stats)
+    {
+        stats->ms_total      = ms;
+        stats->ms_main       = ms;
+        stats->main_launches = 1;
+        stats->passes        = 1;
+        stats->grid_waves    = g->grid;
+    }
+    return finish_stats(stats, g->has_params ? &g->params : nullptr);
+}
+
+int
+loom_graph_destroy(loom_graph* g)
+{
+    if(!g) return 0;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(g->exec) hipGraphExecDestroy(g->exec);
+    if(g->graph) hipGraphDestroy(g->graph);
+    if(g->jobs) hipFree(g->jobs);
+    if(g->ctl) hipFree(g->ctl);
+    delete g;
     return 0;
 }
 
