@@ -30,6 +30,10 @@ METRIC = ("path-trace samples/sec + active-lane % (hybrid vs compact GPU); "
           "p99 req latency")
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published
 W, H, SPP = 1024, 768, 64
+SERVE_QPS = 1000.0
+SERVE_REQUESTS = 1000
+SERVE_BUDGET = 4 << 30
+SERVE_STEP = 4 << 20
 
 
 def parse():
@@ -39,7 +43,69 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compact", action="store_true", help="skip the compact comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job mixed batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--qps", type=float, default=SERVE_QPS, help="serving arrival rate")
+    ap.add_argument("--requests", type=int, default=SERVE_REQUESTS)
     return ap.parse_args()
+
+
+def mixed_batch(steps):
+    """The 64-job mixed batch in one persistent launch replayed from a HIP
+    graph, against one launch per job; outputs compared bit for bit."""
+    import loom_amd
+    from loom_amd import abi, autotune, mixed
+    mb = mixed.MixedBatch()
+    bp = autotune.best_params("batch:" + mixed.SET)
+    params = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=False, **bp)
+    g = mb.batch().graph(params)
+    g.launch()  # warm
+    t = time.perf_counter()
+    launch_ms = 0.0
+    for _ in range(steps):
+        launch_ms += g.launch().ms_main
+    batch_ms = (time.perf_counter() - t) * 1e3 / steps
+    launch_ms /= steps
+    got = mb.outputs()
+    sep_ms = mb.run_separately()
+    want = mb.outputs()
+    same = all(np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                              y.view(np.uint32) if y.dtype == np.float32 else y)
+               for gj, wj in zip(got, want) for x, y in zip(gj, wj))
+    g.destroy()
+    kinds = {}
+    for name, spec in mb.specs:
+        kinds[name] = kinds.get(name, 0) + spec["n"]
+    return {"workload": "64 seeded jobs: frames w,h in [64,1024], spp in [1,64]; fib and "
+                        "Collatz of 2^12-2^22 threads",
+            "threads": mb.threads, "threads_by_kernel": kinds,
+            "ms_per_step": batch_ms, "graph_launch_ms": launch_ms,
+            "threads_per_s": mb.threads / (batch_ms / 1e3),
+            "separate_launches_ms": sep_ms, "speedup_vs_separate": sep_ms / batch_ms,
+            "outputs_identical_to_separate": same, "params": bp}
+
+
+def serving(qps, count):
+    """A Poisson stream of mixed requests served with continuous batching."""
+    from loom_amd import serve
+    pools = serve.Pools.make()
+    specs = serve.poisson_trace(qps=qps, count=count)
+    # outputs land in shared pinned sinks (this run does not read them;
+    # the serving tests check every request's bits)
+    sinks = {"smallpt9_f32": [serve.Pinned(serve.PT_MAX, np.float32) for _ in range(3)],
+             "fib_u64": [serve.Pinned(1 << serve.INT_MAX_LOG2, np.uint64)],
+             "collatz_u64": [serve.Pinned(1 << serve.INT_MAX_LOG2, np.uint32)]}
+    outs = [[p.array[:s.n] for p in sinks[s.kernel]] for s in specs]
+    srv = serve.Server(hbm_budget=SERVE_BUDGET, max_resident=32, step_threads=SERVE_STEP)
+    # warm the path (first launches, page-in of the pools)
+    srv.run(specs[:8], pools, realtime=False, outputs=outs[:8])
+    reqs, st, _ = srv.run(specs, pools, realtime=True, outputs=outs)
+    srv.close()
+    summ = serve.latency_summary(reqs, st)
+    summ.update({"qps": qps, "hbm_budget": SERVE_BUDGET, "step_threads": SERVE_STEP,
+                 "workload": "Poisson arrivals; half frames 128^2-512^2 at 4-16 spp, half "
+                             "fib/Collatz of 2^14-2^20 threads"})
+    return summ
 
 
 def traffic_from_profile():
@@ -76,8 +142,9 @@ def cpu_baseline():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported in this build "
-              f"(the path shards by frame or by sample range; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path "
+              f"would shard by sample range or whole jobs with one process per GPU and no "
+              f"exchange, and only the one-GPU share is built here", file=sys.stderr)
         return 2
     import loom_amd
     from loom_amd import abi, workloads, autotune
@@ -148,6 +215,14 @@ def main():
         "launch_ms": launch_ms, "launches_per_step": launches,
         "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
                 "(fma = 2) per launch over the launch time, against the FP32 vector peak"}
+
+    if not a.no_batch:
+        result["batch"] = mixed_batch(max(1, min(a.steps, 3)))
+    if not a.no_serve:
+        sv = serving(a.qps, a.requests)
+        result["serving"] = sv
+        result["p50_latency_ms"] = sv["p50_ms"]
+        result["p99_latency_ms"] = sv["p99_ms"]
 
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline()

@@ -142,6 +142,15 @@ int loom_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int loom_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int loom_memset(void* dst, int value, size_t bytes);
 int loom_synchronize(void);
+/* Fill `count` elements of `elem_bytes` (1, 2, 4 or 8) bytes at `dst` with
+ * the element at `pattern`: the broadcast of a scalar argument, which the
+ * reference applies when a function argument is a scalar instead of a range
+ * (src/vm/bve.hpp:1125-1133).  A kernel, not a memset. */
+int loom_fill(void* dst, const void* pattern, size_t elem_bytes, uint64_t count);
+/* Pinned host memory for staging (the reference CUDA backend stages through
+ * pinned buffers, src/vm/cuda.hpp:1041-1055). */
+int loom_host_alloc(void** ptr, size_t bytes);
+int loom_host_free(void* ptr);
 
 /* ---- function execution --------------------------------------------------
  * Replaces compact_function<K>::operator() (simd.hpp:3837-4008) and
@@ -188,6 +197,63 @@ int loom_batch_graph_create(int set, const loom_job* jobs, int njobs,
                             const loom_params* params, loom_graph** graph);
 int loom_graph_launch(loom_graph* graph, loom_run_stats* stats);
 int loom_graph_destroy(loom_graph* graph);
+
+/* ---- serving -----------------------------------------------------------------
+ * Continuous batching of kernel launches (requests).  No counterpart in the
+ * reference, whose nearest relative is the Shop's exchange loop that admits
+ * new inputs into a stream as old ones halt (src/vm/mp.hpp:61-131, 316-385).
+ * A request is admitted FIFO when its device columns fit the HBM budget and
+ * a job slot is free; every step is ONE batch launch that advances all
+ * resident requests by a slice of their threads (step_threads shared
+ * among them), so requests that arrive while others run join the next step
+ * and share the device with them.  A request completes at the end of the
+ * step that ran its last slice; its outputs are copied back and its device
+ * memory is released. */
+typedef struct loom_server loom_server;
+
+typedef struct loom_server_config
+{
+    int32_t     batch_set;
+    uint32_t    max_resident; /* requests resident at once, 1..LOOM_MAX_JOBS */
+    uint64_t    hbm_budget;   /* bytes of device columns of resident requests */
+    uint64_t    step_threads; /* threads advanced per step, shared by residents */
+    loom_params params;       /* batch launch parameters */
+} loom_server_config;
+
+typedef struct loom_request
+{
+    int32_t     kernel;                /* global kernel index, member of the set */
+    uint32_t    broadcast;             /* bit i: input i is scalars[i], not a column */
+    uint64_t    n;                     /* threads */
+    const void* inputs[LOOM_MAX_IO];   /* host columns of n elements */
+    void*       outputs[LOOM_MAX_IO];  /* host columns of n elements */
+    uint64_t    scalars[LOOM_MAX_IO];  /* broadcast values (low bytes used) */
+    double      arrival_ms;            /* arrival, relative to the run's start */
+    /* set by the server */
+    double      admit_ms;              /* admission (inputs resident) */
+    double      done_ms;               /* completion (outputs on the host) */
+    uint32_t    steps;                 /* steps the request took part in */
+    int32_t     status;                /* 0, or the error code of the request */
+} loom_request;
+
+typedef struct loom_server_stats
+{
+    double   makespan_ms;  /* run start .. last completion */
+    double   gpu_ms;       /* sum of the batch launches' durations */
+    uint64_t steps;        /* batch launches */
+    uint64_t threads;      /* threads of completed requests */
+    uint32_t peak_resident;
+    uint32_t pad;
+    uint64_t peak_bytes;   /* most device bytes held by resident requests */
+} loom_server_stats;
+
+int loom_server_create(const loom_server_config* config, loom_server** server);
+/* Serve `count` requests sorted by arrival_ms.  realtime != 0 replays the
+ * arrival times against the wall clock (idle until the next arrival);
+ * realtime == 0 treats every request as arrived at the start. */
+int loom_server_run(loom_server* server, loom_request* requests, uint64_t count,
+                    int realtime, loom_server_stats* stats);
+int loom_server_destroy(loom_server* server);
 
 #ifdef __cplusplus
 }

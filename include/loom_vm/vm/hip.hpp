@@ -127,6 +127,27 @@ using input_tuple = typename detail::types_of<typename K::input_vars>::tuple;
 template <Kernel K>
 using output_tuple = typename detail::types_of<typename K::output_vars>::tuple;
 
+namespace detail
+{
+template <typename Tuple>
+struct maybe_single
+{
+    using type = Tuple;
+};
+template <typename T>
+struct maybe_single<std::tuple<T>>
+{
+    using type = T;
+};
+} // namespace detail
+
+/// @brief Row types as the reference spells them (core.hpp:1050-1065): the
+/// bare type for one variable, a tuple otherwise.
+template <Kernel K>
+using input_type = typename detail::maybe_single<input_tuple<K>>::type;
+template <Kernel K>
+using output_type = typename detail::maybe_single<output_tuple<K>>::type;
+
 /// @brief Host columns of a kernel's inputs or outputs (structure of arrays).
 template <typename Tuple>
 struct Columns;
@@ -174,6 +195,26 @@ struct Columns<std::tuple<Ts...>>
     {
         [&]<std::size_t... I>(std::index_sequence<I...>)
         { ((std::get<I>(cols)[i] = std::get<I>(v)), ...); }(std::index_sequence_for<Ts...>{});
+    }
+
+    /// @brief Row in the reference spelling (bare value for one column).
+    auto
+    row_value(std::size_t i) const
+    {
+        if constexpr(sizeof...(Ts) == 1)
+            return std::get<0>(cols)[i];
+        else
+            return row(i);
+    }
+
+    template <typename V>
+    void
+    set_row_value(std::size_t i, const V& v)
+    {
+        if constexpr(sizeof...(Ts) == 1 && !requires { std::tuple_size<V>::value; })
+            std::get<0>(cols)[i] = static_cast<std::tuple_element_t<0, std::tuple<Ts...>>>(v);
+        else
+            set_row(i, std::tuple<Ts...>(v));
     }
 };
 
@@ -267,14 +308,9 @@ class function
     auto
     get_output_zipped() const noexcept
     {
-        if constexpr(std::tuple_size_v<out_t> == 1)
-            return std::get<0>(m_out.cols);
-        else
-        {
-            std::vector<out_t> rows(m_out.size());
-            for(std::size_t i = 0; i < rows.size(); ++i) rows[i] = m_out.row(i);
-            return rows;
-        }
+        std::vector<output_type<K>> rows(m_out.size());
+        for(std::size_t i = 0; i < rows.size(); ++i) rows[i] = m_out.row_value(i);
+        return rows;
     }
 
     void
@@ -319,6 +355,7 @@ class hybrid_stream
 {
     using in_t  = input_tuple<K>;
     using out_t = output_tuple<K>;
+    using row_t = output_type<K>;
 
     std::size_t       m_slots;
     params            m_p;
@@ -346,7 +383,7 @@ class hybrid_stream
         for(std::size_t s = 0; s < m_slots && it != std::ranges::end(input); ++s)
             if(m_state[s] == 0)
             {
-                m_in.set_row(s, in_t(*it));
+                m_in.set_row_value(s, *it);
                 m_state[s] = 1;
                 ++it;
             }
@@ -392,16 +429,16 @@ class hybrid_stream
     /// @brief Outputs of halted slots, each slot refilled with the next row
     /// of @p input; as many as both allow.
     template <std::ranges::input_range R>
-    std::vector<out_t>
+    std::vector<row_t>
     exchange(R&& input) noexcept
     {
-        std::vector<out_t> res;
+        std::vector<row_t> res;
         auto               it = std::ranges::begin(input);
         for(std::size_t s = 0; s < m_slots && it != std::ranges::end(input); ++s)
             if(m_state[s] == 2)
             {
-                res.push_back(m_out.row(s));
-                m_in.set_row(s, in_t(*it));
+                res.push_back(m_out.row_value(s));
+                m_in.set_row_value(s, *it);
                 m_state[s] = 1;
                 ++it;
             }
@@ -415,14 +452,14 @@ class hybrid_stream
     }
 
     /// @brief Outputs still held by halted slots (then the slots are empty).
-    std::vector<out_t>
+    std::vector<row_t>
     get_output_zipped() noexcept
     {
-        std::vector<out_t> res;
+        std::vector<row_t> res;
         for(std::size_t s = 0; s < m_slots; ++s)
             if(m_state[s] == 2)
             {
-                res.push_back(m_out.row(s));
+                res.push_back(m_out.row_value(s));
                 m_state[s] = 0;
             }
         return res;

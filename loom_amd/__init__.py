@@ -123,6 +123,14 @@ class DeviceArray:
     def zero(self) -> None:
         abi.check(abi.load().loom_memset(self.ptr, 0, max(1, self.nbytes)), "memset")
 
+    @classmethod
+    def full(cls, n: int, value, dtype) -> "DeviceArray":
+        """A column of n copies of `value`, filled on the device."""
+        d = cls(n, dtype)
+        v = np.array([value], dtype=d.dtype)
+        abi.check(abi.load().loom_fill(d.ptr, v.ctypes.data, d.dtype.itemsize, d.n), "fill")
+        return d
+
     def free(self) -> None:
         if self.ptr:
             abi.load().loom_free(self.ptr)

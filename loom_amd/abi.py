@@ -27,6 +27,8 @@ C_SPILL_ROWS, C_SPILL_BYTES, C_FILL_ROWS, C_FILL_BYTES = 4, 5, 6, 7
 C_ADMITTED, C_REFILLS, C_BATCHES, C_SPILL_FAILED = 8, 9, 10, 11
 C_ACTIVE_FLOPS = 12
 
+E_ARG, E_KERNEL, E_NODEVICE, E_RUNTIME, E_CAPACITY, E_STATE = range(1000, 1006)
+
 ERR_ASSERT_FP = 1
 ERR_STEP_BUDGET = 2
 
@@ -90,6 +92,44 @@ class Job(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("batch_set", C.c_int32),
+        ("max_resident", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+        ("step_threads", C.c_uint64),
+        ("params", Params),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("broadcast", C.c_uint32),
+        ("n", C.c_uint64),
+        ("inputs", C.c_void_p * MAX_IO),
+        ("outputs", C.c_void_p * MAX_IO),
+        ("scalars", C.c_uint64 * MAX_IO),
+        ("arrival_ms", C.c_double),
+        ("admit_ms", C.c_double),
+        ("done_ms", C.c_double),
+        ("steps", C.c_uint32),
+        ("status", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("makespan_ms", C.c_double),
+        ("gpu_ms", C.c_double),
+        ("steps", C.c_uint64),
+        ("threads", C.c_uint64),
+        ("peak_resident", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("peak_bytes", C.c_uint64),
+    ]
+
+
 # every symbol declared in include/loom_amd.h, with its signature
 SIGNATURES = {
     "loom_init": (C.c_int, [C.c_int]),
@@ -106,6 +146,9 @@ SIGNATURES = {
     "loom_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "loom_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "loom_synchronize": (C.c_int, []),
+    "loom_fill": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64]),
+    "loom_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "loom_host_free": (C.c_int, [C.c_void_p]),
     "loom_run": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                            C.c_uint64, C.POINTER(Params), C.POINTER(RunStats)]),
     "loom_run_host": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p),
@@ -120,6 +163,10 @@ SIGNATURES = {
                                           C.POINTER(C.c_void_p)]),
     "loom_graph_launch": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
     "loom_graph_destroy": (C.c_int, [C.c_void_p]),
+    "loom_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "loom_server_run": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_uint64, C.c_int,
+                                  C.POINTER(ServerStats)]),
+    "loom_server_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

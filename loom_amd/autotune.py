@@ -3,6 +3,7 @@ the regroup switch, the reconvergence policy and the persistent grid
 (waves per CU) by timing candidates on a sample of the real workload.
 
     python -m loom_amd.autotune smallpt9_f32      # tunes, writes tuned.json
+    python -m loom_amd.autotune batch:pt_int_mix  # the 64-job mixed batch
 
 Results are stored per kernel in loom_amd/tuned.json; `best_params` reads
 them (or returns the defaults) so benchmarks and batched launches start from
@@ -71,6 +72,39 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
             "n": n, "candidates": len(results)}
 
 
+BATCH_SPACE = {
+    "live_threshold": [32, 48, 56],
+    "waves_per_cu": [0, 8, 12],
+}
+
+
+def tune_batch(mixed, reps: int = 2, space: dict | None = None, verbose: bool = True) -> dict:
+    """Time every candidate of `space` on a resident mixed batch (one
+    persistent launch per run) and return the best parameters."""
+    import loom_amd
+    from loom_amd import abi
+
+    space = space or BATCH_SPACE
+    keys = list(space)
+    b = mixed.batch()
+    results = []
+    for combo in itertools.product(*(space[k2] for k2 in keys)):
+        cand = {**DEFAULT, **dict(zip(keys, combo))}
+        cand.pop("regroup", None)
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=False, **cand)
+        b.run(p)  # warm
+        ms = 0.0
+        for _ in range(reps):
+            ms += b.run(p).ms_main
+        ms /= reps
+        results.append((ms, cand))
+        if verbose:
+            print(f"tune batch {cand} {ms:.3f} ms", flush=True)
+    results.sort(key=lambda r: r[0])
+    return {"params": {k2: results[0][1][k2] for k2 in keys}, "ms": results[0][0],
+            "threads": mixed.threads, "candidates": len(results)}
+
+
 def save(kernel_name: str, entry: dict) -> None:
     data = json.loads(TUNED.read_text()) if TUNED.exists() else {}
     data[kernel_name] = entry
@@ -81,6 +115,12 @@ def main(argv=None) -> int:
     from loom_amd import workloads
     argv = argv or sys.argv[1:]
     name = argv[0] if argv else "smallpt9_f32"
+    if name.startswith("batch:"):
+        from loom_amd import mixed
+        entry = tune_batch(mixed.MixedBatch())
+        save(name, entry)
+        print(json.dumps({name: entry}), flush=True)
+        return 0
     if name.startswith("smallpt9"):
         ins = workloads.pt_inputs(1024, 768, 8)       # a frame at 1/8 of the spp
     elif name.startswith("collatz"):

@@ -42,7 +42,7 @@ COMMON = [
 
 def sources() -> list[Path]:
     csrc = PKG / "csrc"
-    return [csrc / "runtime.cpp"] + sorted(csrc.glob("kernels_*.hip"))
+    return [csrc / "runtime.cpp", csrc / "serve.cpp"] + sorted(csrc.glob("kernels_*.hip"))
 
 
 def _compile(src: Path) -> Path:
@@ -78,6 +78,26 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+FACADE_SRC = ROOT / "tests" / "cpp" / "facade_test.cpp"
+FACADE_BIN = ROOT / "tests" / "cpp" / "_build" / "facade_test"
+
+
+def build_facade_test() -> Path:
+    """Host program that drives the library through the C++ facade
+    (include/loom_vm/vm/hip.hpp); host compiler only, links the library."""
+    FACADE_BIN.parent.mkdir(parents=True, exist_ok=True)
+    tmp = FACADE_BIN.with_suffix(".tmp")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++20", "-O2", "-Wall",
+           f"-I{ROOT / 'include' / 'loom_vm'}", f"-I{ROOT / 'include'}",
+           str(FACADE_SRC), "-o", str(tmp), f"-L{BUILD}", "-lloom_amd",
+           f"-Wl,-rpath,{BUILD}", "-Wl,-rpath,$ORIGIN/../../../loom_amd/_build"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade test build failed\n{r.stderr[-8000:]}")
+    os.replace(tmp, FACADE_BIN)
+    return FACADE_BIN
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="library only")
@@ -86,6 +106,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if not a.oracle:
         print("built", build_lib(a.j), flush=True)
+        print("built", build_facade_test(), flush=True)
     if not a.lib:
         print("built", build_oracle(), flush=True)
     return 0

@@ -274,6 +274,28 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
     }
     return 0;
 }
+/// @brief Broadcast one element over a column (grid-stride, vector stores).
+template <typename T>
+__global__ void
+fill_kernel(T* dst, T value, std::uint64_t count)
+{
+    const std::uint64_t stride = std::uint64_t(gridDim.x) * blockDim.x;
+    for(std::uint64_t i = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        dst[i] = value;
+}
+
+template <typename T>
+hipError_t
+launch_fill(void* dst, const void* pattern, std::uint64_t count)
+{
+    T v;
+    std::memcpy(&v, pattern, sizeof(T));
+    const std::uint64_t blocks = std::min<std::uint64_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                       g_dev.stream, static_cast<T*>(dst), v, count);
+    return hipGetLastError();
+}
+
 __global__ void
 zero_counters_kernel(std::uint32_t* err, std::uint64_t* stats, std::uint32_t nstats)
 {
@@ -492,6 +514,41 @@ loom_memset(void* dst, int value, size_t bytes)
     if(int r = ensure_device()) return r;
     LOOM_HIP(hipMemsetAsync(dst, value, bytes, g_dev.stream));
     LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_fill(void* dst, const void* pattern, size_t elem_bytes, uint64_t count)
+{
+    if(int r = ensure_device()) return r;
+    if(!count) return 0;
+    if(!dst || !pattern) return fail(LOOM_E_ARG, "null pointer");
+    switch(elem_bytes)
+    {
+    case 1: LOOM_HIP(launch_fill<std::uint8_t>(dst, pattern, count)); break;
+    case 2: LOOM_HIP(launch_fill<std::uint16_t>(dst, pattern, count)); break;
+    case 4: LOOM_HIP(launch_fill<std::uint32_t>(dst, pattern, count)); break;
+    case 8: LOOM_HIP(launch_fill<std::uint64_t>(dst, pattern, count)); break;
+    default: return fail(LOOM_E_ARG, "element size %zu is not 1, 2, 4 or 8", elem_bytes);
+    }
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
+loom_host_alloc(void** ptr, size_t bytes)
+{
+    if(int r = ensure_device()) return r;
+    if(!ptr) return fail(LOOM_E_ARG, "null pointer");
+    LOOM_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return 0;
+}
+
+int
+loom_host_free(void* ptr)
+{
+    if(int r = ensure_device()) return r;
+    LOOM_HIP(hipHostFree(ptr));
     return 0;
 }
 

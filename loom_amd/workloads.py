@@ -68,3 +68,35 @@ def pt_image(lx, ly, lz, w: int, h: int, spp: int) -> np.ndarray:
             acc = (acc + s[:, k]).astype(np.float32)
         out[:, c] = acc / np.float32(spp)
     return out.reshape(h, w, 3)
+
+
+def batch_jobs(njobs: int = 64, seed: int = SEED + 2, scale: int = 1):
+    """The mixed batch: `njobs` job specs drawn from `seed`.
+
+    Even jobs are path-trace frames with width and height uniform in
+    [64, 1024] and spp uniform in [1, 64]; odd jobs alternate fib and
+    Collatz with 2^k threads, k uniform in [12, 22].  `scale` > 1 divides
+    every size (for tests).  Returns a list of (kernel name, spec dict)."""
+    r = uniform_int(seed, 4 * njobs, 0, (1 << 32) - 1).astype(np.uint64)
+    jobs = []
+    for j in range(njobs):
+        a, b, c, d = (int(x) for x in r[4 * j:4 * j + 4])
+        if j % 2 == 0:
+            w = max(1, (64 + a % 961) // scale)
+            h = max(1, (64 + b % 961) // scale)
+            spp = max(1, (1 + c % 64) // scale)
+            jobs.append(("smallpt9_f32", {"w": w, "h": h, "spp": spp, "n": w * h * spp}))
+        else:
+            n = max(1, (1 << (12 + a % 11)) // scale)
+            name = "fib_u64" if j % 4 == 1 else "collatz_u64"
+            jobs.append((name, {"n": n, "seed": seed + 1000 + j}))
+    return jobs
+
+
+def job_inputs(name: str, spec: dict):
+    """Host input columns of one batch job spec."""
+    if name.startswith("smallpt9"):
+        return pt_inputs(spec["w"], spec["h"], spec["spp"])
+    if name.startswith("fib"):
+        return [fib_inputs(spec["n"], seed=spec["seed"])]
+    return [collatz_inputs(spec["n"], seed=spec["seed"])]

@@ -1,0 +1,62 @@
+"""Serving: a mixed Poisson request stream served with continuous batching
+gives every request the same bits as the same launch run alone, while
+requests share steps on the device; admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+import loom_amd
+from loom_amd import abi, serve
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+@pytest.fixture(scope="module")
+def pools(gpu):
+    return serve.Pools.make()
+
+
+def test_mixed_stream_equals_alone(pools):
+    specs = serve.poisson_trace(qps=4000.0, count=48, scale=4, seed=11)
+    srv = serve.Server(hbm_budget=48 << 20, max_resident=8, step_threads=1 << 16)
+    print(f"[gpu] serve {len(specs)} requests realtime", flush=True)
+    reqs, st, outs = srv.run(specs, pools, realtime=True)
+    srv.close()
+    assert all(q.status == 0 for q in reqs)
+    # the mix really overlapped: several requests resident in one step,
+    # and requests spread over several steps
+    assert st.peak_resident > 1
+    assert max(q.steps for q in reqs) > 1
+    assert st.peak_bytes <= 48 << 20
+    for s, q, o in zip(specs, reqs, outs):
+        assert q.done_ms >= q.admit_ms >= q.arrival_ms - 1e-6
+        alone = loom_amd.run(s.kernel, abi.MODE_HYBRID, pools.inputs_of(s, full=True)).outputs
+        for g, a in zip(o, alone):
+            assert np.array_equal(bits(g), bits(a)), (s.kernel, s.n)
+    # and the oracle agrees on a few
+    for s, o in list(zip(specs, outs))[:6]:
+        want, err = O.run(s.kernel, pools.inputs_of(s, full=True))
+        assert err == 0
+        for g, w in zip(o, want):
+            assert np.array_equal(bits(g), bits(w)), (s.kernel, s.n)
+    summ = serve.latency_summary(reqs, st)
+    print(summ)
+    assert summ["p99_ms"] >= summ["p50_ms"] > 0
+
+
+def test_budget_rejects_oversized_request(pools):
+    specs = [serve.RequestSpec("collatz_u64", 5000, 0.0, offset=7),
+             serve.RequestSpec("collatz_u64", 1 << 20, 0.0, offset=0),  # 12 MiB > budget
+             serve.RequestSpec("fib_u64", 3000, 0.0, offset=100)]
+    srv = serve.Server(hbm_budget=1 << 20, max_resident=4, step_threads=1 << 12)
+    print("[gpu] serve budget rejection", flush=True)
+    reqs, st, outs = srv.run(specs, pools, realtime=False)
+    srv.close()
+    assert [q.status for q in reqs] == [0, abi.E_CAPACITY, 0]
+    for i in (0, 2):
+        want, _ = O.run(specs[i].kernel, pools.inputs_of(specs[i]))
+        assert np.array_equal(outs[i][0], want[0])
