@@ -1,0 +1,141 @@
+"""Writes the golden fixtures of this directory.
+
+    python tests/golden/make_golden.py
+
+The reference holds no golden vectors for this path and cannot be built
+here, so the fixtures come from computations independent of the oracle's
+block interpreter and of the GPU backend:
+
+* fib_u64_c1.json — the fib workload (2^20 seeded inputs in [0, 90]); the
+  expected outputs come from the Fibonacci recurrence in Python integers.
+* collatz_u64_c2.json — the Collatz workload (2^24 seeded inputs in
+  [1, 2^32)); the expected step counts come from a vectorised brute-force
+  iteration in numpy with an overflow check.
+* smallpt9_32x24x8.npz — a 32x24 frame at 8 spp of the 9-sphere path
+  tracer from the direct C++ restatement (not the IR): per-sample radiance
+  bits and the per-pixel image.
+* literals.json — literal strings with the correctly rounded float32 and
+  float64 values (Python's parser) beside the values of the reference's
+  literal parser as restated by the oracle.
+
+Each file stores SHA-256 digests of the full input and output columns and a
+sample of (index, input, output) rows.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import struct
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+
+from loom_amd import workloads  # noqa: E402
+
+SAMPLE = 1024
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def sample_rows(n: int, seed: int) -> np.ndarray:
+    idx = workloads.uniform_int(seed, SAMPLE, 0, n - 1).astype(np.int64)
+    return np.unique(np.concatenate([idx, [0, n - 1]]))
+
+
+def fib_fixture():
+    n = workloads.fib_inputs(1 << 20)
+    table = [0, 1]
+    while len(table) < 91:
+        table.append(table[-1] + table[-2])
+    assert table[90] < 1 << 64
+    c = np.array([table[int(v)] for v in range(91)], dtype=np.uint64)[n.astype(np.int64)]
+    rows = sample_rows(len(n), 101)
+    return {"kernel": "fib_u64", "threads": len(n), "inputs_sha256": sha(n),
+            "outputs_sha256": sha(c),
+            "sample": [[int(i), int(n[i]), int(c[i])] for i in rows],
+            "how": "Fibonacci recurrence in Python integers"}
+
+
+def collatz_steps(x: np.ndarray) -> np.ndarray:
+    """Step counts of the Collatz map until 1, vectorised, with an overflow
+    check on 3x+1."""
+    x = x.astype(np.uint64).copy()
+    steps = np.zeros(len(x), np.uint32)
+    idx = np.flatnonzero(x > 1)
+    cur = x[idx]
+    limit = np.uint64((2**64 - 1 - 1) // 3)
+    while len(idx):
+        odd = (cur & np.uint64(1)).astype(bool)
+        if (cur[odd] > limit).any():
+            raise OverflowError("3x+1 overflows u64")
+        cur = np.where(odd, cur * np.uint64(3) + np.uint64(1), cur >> np.uint64(1))
+        steps[idx] += 1
+        keep = cur > 1
+        idx, cur = idx[keep], cur[keep]
+    return steps
+
+
+def collatz_fixture():
+    x = workloads.collatz_inputs(1 << 24)
+    s = collatz_steps(x)
+    rows = sample_rows(len(x), 202)
+    return {"kernel": "collatz_u64", "threads": len(x), "inputs_sha256": sha(x),
+            "outputs_sha256": sha(s), "max_steps": int(s.max()), "sum_steps": int(s.sum(dtype=np.uint64)),
+            "sample": [[int(i), int(x[i]), int(s[i])] for i in rows],
+            "how": "vectorised brute-force Collatz iteration (numpy), overflow-checked"}
+
+
+def smallpt_fixture():
+    from tests import oracle_lib as O
+    w, h, spp = 32, 24, 8
+    sid, job = workloads.pt_inputs(w, h, spp)
+    lx, ly, lz = O.smallpt9_direct(sid, job)
+    img = workloads.pt_image(lx, ly, lz, w, h, spp)
+    np.savez_compressed(HERE / "smallpt9_32x24x8.npz",
+                        w=w, h=h, spp=spp, job=np.uint32(job[0]),
+                        lx=lx.view(np.uint32), ly=ly.view(np.uint32), lz=lz.view(np.uint32),
+                        image=img)
+
+
+LITERALS = ["0", "1", "0.5", "0.1", "0.25", "2.5", "1e-12", "1e-6", "1e5", "1e30", "-0.1",
+            "6.283185307179586", "3.14159265", "0.093073", "0.999", "81.6", "681.33",
+            "0.042612", "0.5135", "295.6", "1.5", "0.75", "12", "16777216", "16777217",
+            "0.0001", "2.718281828", "100000.5", "1e-3", "0.333333333", "7.5e-4", "1e1"]
+
+
+def literal_fixture():
+    from tests import oracle_lib as O
+    out = []
+    for t in LITERALS:
+        f32 = O.parse("f32", t)
+        f64 = O.parse("f64", t)
+        out.append({"text": t,
+                    "ref_f32_bits": struct.unpack("<I", struct.pack("<f", f32))[0],
+                    "ref_f64_bits": struct.unpack("<Q", struct.pack("<d", f64))[0],
+                    "rn_f32_bits": struct.unpack("<I", struct.pack("<f", float(np.float32(t))))[0],
+                    "rn_f64_bits": struct.unpack("<Q", struct.pack("<d", float(t)))[0]})
+    return {"literals": out,
+            "how": "ref_*: the reference literal parser as restated by the oracle; "
+                   "rn_*: correctly rounded (Python/numpy parsers)"}
+
+
+def main():
+    (HERE / "fib_u64_c1.json").write_text(json.dumps(fib_fixture(), indent=0) + "\n")
+    print("fib", flush=True)
+    (HERE / "collatz_u64_c2.json").write_text(json.dumps(collatz_fixture(), indent=0) + "\n")
+    print("collatz", flush=True)
+    smallpt_fixture()
+    print("smallpt9", flush=True)
+    (HERE / "literals.json").write_text(json.dumps(literal_fixture(), indent=0) + "\n")
+    print("literals", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""The committed golden fixtures (tests/golden/, written by make_golden.py
+from computations independent of the oracle's interpreter) against the
+workload generator and the oracle."""
+import hashlib
+import json
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from loom_amd import workloads
+from tests import oracle_lib as O
+
+G = Path(__file__).resolve().parent / "golden"
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load(name):
+    return json.loads((G / name).read_text())
+
+
+def test_fib_workload_and_oracle():
+    f = load("fib_u64_c1.json")
+    n = workloads.fib_inputs(f["threads"])
+    assert sha(n) == f["inputs_sha256"]  # generator is stable across machines
+    (c,), err = O.run("fib_u64", [n])
+    assert err == 0
+    assert sha(c) == f["outputs_sha256"]
+
+
+def test_collatz_workload_and_oracle_sample():
+    f = load("collatz_u64_c2.json")
+    x = workloads.collatz_inputs(f["threads"])
+    assert sha(x) == f["inputs_sha256"]
+    rows = np.array(f["sample"], dtype=np.uint64)
+    assert np.array_equal(x[rows[:, 0].astype(np.int64)], rows[:, 1])
+    (s,), err = O.run("collatz_u64", [rows[:, 1]])
+    assert err == 0
+    assert np.array_equal(s.astype(np.uint64), rows[:, 2])
+
+
+def test_smallpt9_frame_oracle_interpreter():
+    """The IR interpreter reproduces the fixture frame (made by the direct
+    C++ restatement) bit for bit."""
+    z = np.load(G / "smallpt9_32x24x8.npz")
+    w, h, spp = int(z["w"]), int(z["h"]), int(z["spp"])
+    ins = workloads.pt_inputs(w, h, spp)
+    assert int(ins[1][0]) == int(z["job"])
+    out, err = O.run("smallpt9_f32", ins)
+    assert err == 0
+    for got, key in zip(out, ("lx", "ly", "lz")):
+        assert np.array_equal(got.view(np.uint32), z[key])
+    img = workloads.pt_image(*(z[k].view(np.float32) for k in ("lx", "ly", "lz")), w, h, spp)
+    assert np.array_equal(img, z["image"])
+
+
+def test_literals_table():
+    f = load("literals.json")
+    for row in f["literals"]:
+        t = row["text"]
+        f32 = O.parse("f32", t)
+        f64 = O.parse("f64", t)
+        assert struct.unpack("<I", struct.pack("<f", f32))[0] == row["ref_f32_bits"], t
+        assert struct.unpack("<Q", struct.pack("<d", f64))[0] == row["ref_f64_bits"], t
+        # the reference parser stays within a few ulp of correct rounding
+        assert abs(row["ref_f32_bits"] - row["rn_f32_bits"]) <= 4, t
+        assert abs(row["ref_f64_bits"] - row["rn_f64_bits"]) <= 8, t
+
+
+@pytest.mark.parametrize("text,exact", [("0.5", True), ("0.25", True), ("12", True),
+                                        ("16777216", True), ("1e5", True)])
+def test_literals_exact_where_representable(text, exact):
+    row = next(r for r in load("literals.json")["literals"] if r["text"] == text)
+    assert (row["ref_f32_bits"] == row["rn_f32_bits"]) == exact
