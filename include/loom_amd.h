@@ -107,6 +107,9 @@ typedef struct loom_params
     uint64_t queue_capacity; /* hybrid rows per label queue; 0 = automatic */
     uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40 */
 } loom_params;
+/* A NULL loom_params* means: threshold 0, refill on, regroup off, no stats,
+ * successor-major for compact, topological with live_threshold 48 for
+ * hybrid. */
 
 typedef struct loom_run_stats
 {

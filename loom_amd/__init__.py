@@ -160,14 +160,18 @@ class RunResult:
         return self.counters[abi.C_ACTIVE_OPS] / slot if slot else float("nan")
 
 
-def make_params(threshold=0, live_threshold=0, refill=True, regroup=True, waves_per_cu=0,
+def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, waves_per_cu=0,
                 queue_capacity=0, max_steps=0, stats=True, reconv=None,
                 mode=abi.MODE_COMPACT) -> abi.Params:
     """Run parameters; `reconv` defaults to successor-major for the compact
-    form (the reference policy) and to the topological policy for hybrid."""
+    form (the reference policy) and to the topological policy for hybrid,
+    `live_threshold` to 48 for hybrid (0 for compact, which does not
+    refill)."""
     p = abi.Params()
     if reconv is None:
         reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_TOPO
+    if live_threshold is None:
+        live_threshold = 48 if mode == abi.MODE_HYBRID else 0
     p.reconv = int(reconv)
     p.threshold = int(threshold)
     p.live_threshold = int(live_threshold)
@@ -248,7 +252,7 @@ class hybrid_function(compact_function):
 
     mode = abi.MODE_HYBRID
 
-    def __init__(self, k, threshold=0, live_threshold=48, refill=True, regroup=True,
+    def __init__(self, k, threshold=0, live_threshold=48, refill=True, regroup=False,
                  waves_per_cu=0, queue_capacity=0, **params):
         super().__init__(k, threshold=threshold, live_threshold=live_threshold, refill=refill,
                          regroup=regroup, waves_per_cu=waves_per_cu,

@@ -73,8 +73,10 @@ run_job(const BatchArgs& a, std::uint32_t j, const CtlArgs& c, WaveCtx& w, std::
 {
     const JobDesc*      J   = a.jobs;
     const std::uint64_t len = J[j].end - J[j].begin;
-    // chunk: about 1/64 of the job per claim, one batch at least, 512 at most
-    std::uint32_t chunk = static_cast<std::uint32_t>(len / 64 < 64 ? 64 : (len / 64 > 512 ? 512 : len / 64));
+    // chunk: the job spread over about four claims per wave that starts on
+    // it (gridDim.x / njobs waves do), one batch at least, 512 at most
+    const std::uint64_t per = len * a.njobs / (4ull * gridDim.x);
+    const std::uint32_t chunk = static_cast<std::uint32_t>(per < 64 ? 64 : (per > 512 ? 512 : per));
     std::uint64_t rbase = 0;
     std::uint32_t rcount = 0;
     bool          exhausted = false;

@@ -734,16 +734,19 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     const std::uint32_t lane  = lane_id();
     const std::uint32_t home  = blockIdx.x % kRanges;
 
+    // useful waves: queue batches plus input batches left
+    std::uint64_t left = 0;
+    for(std::uint32_t i = 0; i < kRanges; ++i)
     {
-        // useful waves: queue batches plus input batches left
-        std::uint64_t left = 0;
-        for(std::uint32_t i = 0; i < kRanges; ++i)
-        {
-            const std::uint64_t nx = ctl->next[i], e = ctl->end[i];
-            left += nx < e ? (e - nx + 63) / 64 : 0;
-        }
-        if(std::uint64_t(blockIdx.x) >= nbq + uniform64(left)) return;
+        const std::uint64_t nx = ctl->next[i], e = ctl->end[i];
+        left += nx < e ? (e - nx + 63) / 64 : 0;
     }
+    left = uniform64(left);
+    if(std::uint64_t(blockIdx.x) >= nbq + left) return;
+    // drain pass: the input is exhausted and every queued batch has a wave
+    // of its own, so regrouping again cannot fill waves better than this
+    // pass does; survivors finish in place (no pass per remaining block)
+    const bool drain = left == 0 && nbq <= gridDim.x;
 
     const char*    qin  = h.q[par];
     char*          qout = h.q[par ^ 1u];
@@ -839,7 +842,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                     continue;
                 }
             }
-            if(c.regroup)
+            if(c.regroup && !drain)
             {
                 // spill survivors into the queues of their labels
                 std::uint64_t todo = live;

@@ -583,9 +583,9 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
     c.n         = n;
     c.max_steps = params && params->max_steps ? params->max_steps : (1ull << 40);
     c.thr       = params ? params->threshold : 0;
-    c.live_thr  = params ? params->live_threshold : 0;
+    c.live_thr  = params ? params->live_threshold : (mode == LOOM_MODE_HYBRID ? 48 : 0);
     c.refill    = params ? params->refill : 1;
-    c.regroup   = params ? params->regroup : 1;
+    c.regroup   = params ? params->regroup : 0;
     c.err       = g_dev.err;
     c.stats     = params && params->collect_stats ? g_dev.stats : nullptr;
     LOOM_HIP(hipMemsetAsync(g_dev.err, 0, sizeof(std::uint32_t), g_dev.stream));

@@ -15,18 +15,21 @@ pytestmark = pytest.mark.gpu
 MODES = [
     ("compact", abi.MODE_COMPACT, {}),
     ("compact_topo", abi.MODE_COMPACT, dict(reconv=abi.RECONV_TOPO)),
-    ("hybrid_t0", abi.MODE_HYBRID, dict(threshold=0, reconv=abi.RECONV_SUCCESSOR_MAJOR)),
-    ("hybrid_topo_live48", abi.MODE_HYBRID, dict(live_threshold=48)),
-    ("hybrid_sm_live48", abi.MODE_HYBRID, dict(live_threshold=48,
+    ("hybrid_t0", abi.MODE_HYBRID, dict(threshold=0, live_threshold=0, regroup=True,
+                                        reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("hybrid_topo_live48", abi.MODE_HYBRID, dict(live_threshold=48, regroup=True)),
+    ("hybrid_sm_live48", abi.MODE_HYBRID, dict(live_threshold=48, regroup=True,
                                                reconv=abi.RECONV_SUCCESSOR_MAJOR)),
     ("hybrid_live60_noregroup", abi.MODE_HYBRID, dict(live_threshold=60, regroup=False)),
-    ("hybrid_t56_norefill", abi.MODE_HYBRID, dict(threshold=56, refill=False,
+    ("hybrid_default", abi.MODE_HYBRID, {}),
+    ("hybrid_t56_norefill", abi.MODE_HYBRID, dict(threshold=56, live_threshold=0, refill=False,
+                                                  regroup=True,
                                                   reconv=abi.RECONV_SUCCESSOR_MAJOR)),
-    ("hybrid_t48_smallq", abi.MODE_HYBRID, dict(threshold=48, live_threshold=40,
+    ("hybrid_t48_smallq", abi.MODE_HYBRID, dict(threshold=48, live_threshold=40, regroup=True,
                                                 queue_capacity=64, waves_per_cu=1,
                                                 reconv=abi.RECONV_SUCCESSOR_MAJOR)),
-    ("hybrid_topo_smallq", abi.MODE_HYBRID, dict(live_threshold=40, queue_capacity=64,
-                                                 waves_per_cu=1)),
+    ("hybrid_topo_smallq", abi.MODE_HYBRID, dict(live_threshold=40, regroup=True,
+                                                 queue_capacity=64, waves_per_cu=1)),
 ]
 STEP_BUDGET = 1 << 24  # every kernel here finishes far below it
 
@@ -99,7 +102,8 @@ FIB = np.array(FIB, dtype=np.uint64)
 
 
 @pytest.mark.parametrize("mode,params", [(abi.MODE_COMPACT, {}),
-                                         (abi.MODE_HYBRID, dict(threshold=32))])
+                                         (abi.MODE_HYBRID, dict(threshold=32, regroup=True)),
+                                         (abi.MODE_HYBRID, {})])
 def test_fib_full_config(gpu, mode, params):
     """fib config: 2^20 threads, n ~ U[0,90], bit-exact vs the closed form."""
     n = workloads.fib_inputs(1 << 20)
@@ -121,7 +125,7 @@ def test_collatz_full_config_forms_agree(gpu):
     a seeded sample matches brute force, and the first 2^16 match the oracle."""
     x = workloads.collatz_inputs(1 << 24)
     a = gpu_run("collatz_u64", abi.MODE_COMPACT, {}, [x]).outputs[0]
-    b = gpu_run("collatz_u64", abi.MODE_HYBRID, dict(threshold=32), [x]).outputs[0]
+    b = gpu_run("collatz_u64", abi.MODE_HYBRID, dict(threshold=32, regroup=True), [x]).outputs[0]
     assert np.array_equal(a, b)
     idx = workloads.uniform_int(99, 400, 0, (1 << 24) - 1).astype(np.int64)
     assert [int(a[i]) for i in idx] == [collatz_ref(int(x[i])) for i in idx]
@@ -142,3 +146,23 @@ def test_smallpt9_frame_vs_oracle(gpu, mname, mode, params):
         assert same(g, o), mname
     img = workloads.pt_image(*got, w, h, spp)
     assert np.isfinite(img).all() and (img >= 0).all()
+
+
+@pytest.mark.parametrize("name,n", [("collatz_u64", 1 << 18), ("smallpt9_f32", 40 * 30 * 8),
+                                    ("fib_u64", 100000)])
+@pytest.mark.parametrize("live", [0, 48])
+def test_regroup_conserves_threads(gpu, name, n, live):
+    """Queue regrouping moves threads, never loses or duplicates them: every
+    spilled row is filled again, every input thread is admitted once, and
+    the outputs equal the in-place run's."""
+    ins = CASES[name](n)
+    a = gpu_run(name, abi.MODE_HYBRID, dict(live_threshold=live, regroup=True,
+                                            waves_per_cu=2, queue_capacity=1 << 16), ins)
+    b = gpu_run(name, abi.MODE_HYBRID, dict(live_threshold=live, regroup=False), ins)
+    for x, y in zip(a.outputs, b.outputs):
+        assert same(x, y)
+    c = a.counters
+    assert c[abi.C_SPILL_ROWS] == c[abi.C_FILL_ROWS]
+    assert c[abi.C_ADMITTED] == n
+    assert c[abi.C_SPILL_FAILED] == 0
+    assert c[abi.C_THREAD_STEPS] == b.counters[abi.C_THREAD_STEPS]
