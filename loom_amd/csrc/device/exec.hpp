@@ -166,19 +166,20 @@ exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
     constexpr ir::Instr   in = ir::table_of<K>.instr[I];
 
     if constexpr(in.kind == SK::jump)
-        pc = ir::table_of<K>.target[in.t0];
+        pc = LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0]));
     else if constexpr(in.kind == SK::pjump)
     {
         const auto v = operand<K, in.a0>(r);
         if constexpr(in.nt == 2)
-            pc = v != decltype(v)(0) ? std::uint32_t(ir::table_of<K>.target[in.t0 + 1]) :
-                                       std::uint32_t(ir::table_of<K>.target[in.t0]);
+            pc = v != decltype(v)(0) ? LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0 + 1])) :
+                                       LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0]));
         else
         {
             std::uint32_t p = HALT;
             [&]<std::size_t... J>(std::index_sequence<J...>)
             {
-                ((p = (v == decltype(v)(J)) ? std::uint32_t(ir::table_of<K>.target[in.t0 + J]) : p), ...);
+                ((p = (v == decltype(v)(J)) ? LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0 + J])) : p),
+                 ...);
             }(std::make_index_sequence<in.nt>{});
             pc = p;
         }
@@ -220,7 +221,8 @@ exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
             E       v = E(0);
             [&]<std::size_t... J>(std::index_sequence<J...>)
             {
-                ((v = (i == IX(J)) ? const_as<E>(ir::table_of<K>.konst[in.c0 + J]) : v), ...);
+                ((v = (i == IX(J)) ? const_as<E>(LOOM_CX(std::uint64_t(ir::table_of<K>.konst[in.c0 + J]))) : v),
+                 ...);
             }(std::make_index_sequence<in.nc>{});
             r.template set<d.slot, D>(conv<D>(v));
         }

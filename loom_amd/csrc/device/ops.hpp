@@ -21,6 +21,10 @@
 
 #ifndef LOOM_DEVICE
 #define LOOM_DEVICE __device__ __forceinline__
+
+/// A compile-time value of the lowered tables used as an immediate: forces
+/// constant evaluation so device code never loads table entries from memory.
+#define LOOM_CX(...) (::std::integral_constant<decltype(__VA_ARGS__), (__VA_ARGS__)>::value)
 #endif
 
 namespace loom::hipvm

@@ -350,14 +350,18 @@ topo_pick(std::uint64_t present)
     std::uint64_t elig = present;
     [&]<std::size_t... H>(std::index_sequence<H...>)
     {
-        ((elig &= (present & ir::table_of<K>.body[H]) ? ~(1ull << ir::table_of<K>.head[H]) : ~0ull),
+        ((elig &= (present & LOOM_CX(std::uint64_t(ir::table_of<K>.body[H])))
+                      ? ~(1ull << LOOM_CX(std::uint32_t(ir::table_of<K>.head[H])))
+                      : ~0ull),
          ...);
     }(std::make_index_sequence<ir::table_of<K>.nhead>{});
     if(!elig) elig = present;
     std::uint32_t best = NONE;
     [&]<std::size_t... R>(std::index_sequence<R...>)
     {
-        (((elig >> ir::table_of<K>.order[R]) & 1u ? (best = ir::table_of<K>.order[R], true) : false) ||
+        (((elig >> LOOM_CX(std::uint32_t(ir::table_of<K>.order[R]))) & 1u
+              ? (best = LOOM_CX(std::uint32_t(ir::table_of<K>.order[R])), true)
+              : false) ||
          ...);
     }(std::make_index_sequence<ir::table_of<K>.nb>{});
     return best;
@@ -563,7 +567,7 @@ LOOM_DEVICE void
 spill_row(const Regs<K>& r, std::uint32_t tid, char* q, std::uint64_t qcap, std::uint32_t row)
 {
     using QL                = QueueLayout<K>;
-    char* base              = q + qcap * QL::rows[B];
+    char* base              = q + qcap * LOOM_CX(QL::rows[B]);
     reinterpret_cast<std::uint32_t*>(base)[row] = tid;
     [&]<std::size_t... S>(std::index_sequence<S...>)
     {
@@ -572,7 +576,7 @@ spill_row(const Regs<K>& r, std::uint32_t tid, char* q, std::uint64_t qcap, std:
             {
                 if constexpr(ir::table_of<K>.block[B].live_in.test(S))
                 {
-                    char* col = base + qcap * QL::col(B, S);
+                    char* col = base + qcap * LOOM_CX(QL::col(B, S));
                     if constexpr(ir::table_of<K>.reg[S].width == 8)
                         reinterpret_cast<std::uint64_t*>(col)[row] = r.template raw<S>();
                     else
@@ -590,7 +594,7 @@ LOOM_DEVICE std::uint32_t
 fill_row(Regs<K>& r, const char* q, std::uint64_t qcap, std::uint32_t row)
 {
     using QL                = QueueLayout<K>;
-    const char* base        = q + qcap * QL::rows[B];
+    const char* base        = q + qcap * LOOM_CX(QL::rows[B]);
     const std::uint32_t tid = reinterpret_cast<const std::uint32_t*>(base)[row];
     [&]<std::size_t... S>(std::index_sequence<S...>)
     {
@@ -599,7 +603,7 @@ fill_row(Regs<K>& r, const char* q, std::uint64_t qcap, std::uint32_t row)
             {
                 if constexpr(ir::table_of<K>.block[B].live_in.test(S))
                 {
-                    const char* col = base + qcap * QL::col(B, S);
+                    const char* col = base + qcap * LOOM_CX(QL::col(B, S));
                     if constexpr(ir::table_of<K>.reg[S].width == 8)
                         r.template set_raw<S>(reinterpret_cast<const std::uint64_t*>(col)[row]);
                     else
@@ -642,6 +646,22 @@ spill_label(std::uint32_t label, const Regs<K>& r, std::uint32_t tid, char* q,
         break;
     LOOM_LABEL_SWITCH(0)
 #undef LOOM_LC
+}
+
+/// @brief Row bytes of label @p label (a switch over compile-time values, so
+/// no table is read from memory).
+template <typename K>
+LOOM_DEVICE std::uint32_t
+label_bytes(std::uint32_t label)
+{
+#define LOOM_LC(b)                                                              \
+    case b:                                                                     \
+        if constexpr(b < ir::table_of<K>.nb)                                    \
+            return LOOM_CX(QueueLayout<K>::live_bytes(b < ir::table_of<K>.nb ? b : 0)); \
+        break;
+    LOOM_LABEL_SWITCH(0)
+#undef LOOM_LC
+    return 0;
 }
 
 template <typename K>
@@ -727,7 +747,6 @@ template <typename K, int P>
 __global__ __launch_bounds__(64) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
-    using QL = QueueLayout<K>;
     HybridCtl* ctl            = h.ctl;
     const std::uint32_t par   = uniform(ctl->parity);
     const std::uint32_t nbq   = uniform(ctl->nbatches);
@@ -787,7 +806,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                 has = true;
             }
             fill_rows += rows;
-            fill_bytes += std::uint64_t(rows) * QL::live_bytes(label);
+            fill_bytes += std::uint64_t(rows) * label_bytes<K>(label);
         }
         else
         {
@@ -807,10 +826,14 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         ++batches;
 
         // ---- execute, refill, spill ----
-        bool force = true;
+        // One call site of the block code (it is inlined: a second site
+        // would double the kernel's instruction footprint).  `finish` runs
+        // the remaining lanes to HALT in place.
+        bool force = true, finish = false;
         for(;;)
         {
-            run_policy<K, P>(r, pc, c.thr, c.live_thr, force, w);
+            run_policy<K, P>(r, pc, finish ? 0u : c.thr, finish ? 0u : c.live_thr, force || finish,
+                             w);
             force = false;
             if(w.over) break;
             if(has && pc == HALT)
@@ -819,7 +842,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                 has = false;
             }
             const std::uint64_t live = ballot(has);
-            if(!live) break;
+            if(!live || finish) break;
             if(c.refill && ~live)
             {
                 const std::uint64_t empty = ~live;
@@ -867,19 +890,13 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                         base >= h.qcap ? 0u :
                                          (h.qcap - base < cntm ? std::uint32_t(h.qcap - base) : cntm);
                     spill_rows += ok;
-                    spill_bytes += std::uint64_t(ok) * QL::live_bytes(label);
+                    spill_bytes += std::uint64_t(ok) * label_bytes<K>(label);
                     spill_failed += cntm - ok;
                 }
                 if(!ballot(has)) break;
             }
             // no regroup (or a full queue): finish the remaining lanes in place
-            run_policy<K, P>(r, pc, 0, 0, true, w);
-            if(has && pc == HALT)
-            {
-                store_outputs<K>(r, io.out, tid);
-                has = false;
-            }
-            break;
+            finish = true;
         }
         if(w.over) break;
     }
