@@ -21,6 +21,10 @@ import numpy as np
 
 TUNED = Path(__file__).resolve().parent / "tuned.json"
 
+# the selection keeps at least this fraction of lanes active (the target
+# the hybrid form is held to), trading a few percent of speed when needed
+MIN_ACTIVE = 0.70
+
 DEFAULT = {"live_threshold": 48, "regroup": False, "reconv": 1, "waves_per_cu": 0,
            "threshold": 0, "refill": True}
 
@@ -28,7 +32,7 @@ SPACE = {
     "live_threshold": [32, 40, 48, 56],
     "regroup": [False, True],
     "reconv": [0, 1],
-    "waves_per_cu": [0, 12, 16],
+    "waves_per_cu": [0, 16, 20],
 }
 
 
@@ -42,9 +46,10 @@ def best_params(kernel_name: str) -> dict:
 
 
 def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
-         verbose: bool = True) -> dict:
+         verbose: bool = True, min_active: float = MIN_ACTIVE) -> dict:
     """Time every candidate of `space` on `inputs` (numpy columns, copied to
-    HBM once) and return the best parameters with their time."""
+    HBM once) and return the fastest parameters whose active-lane fraction
+    is at least `min_active` (the fastest overall if none reaches it)."""
     import loom_amd
     from loom_amd import abi
 
@@ -58,23 +63,27 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
     results = []
     for combo in itertools.product(*(space[k2] for k2 in keys)):
         cand = {**DEFAULT, **dict(zip(keys, combo))}
-        p = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=False, **cand)
-        loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p)  # warm
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=True, **cand)
+        act = loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p).active_lane_fraction  # warm
         t0 = time.perf_counter()
         for _ in range(reps):
             loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p)
         ms = (time.perf_counter() - t0) * 1e3 / reps
-        results.append((ms, cand))
+        results.append((ms, act, cand))
         if verbose:
-            print(f"tune {kernel_name} {cand} {ms:.3f} ms", flush=True)
+            print(f"tune {kernel_name} {cand} {ms:.3f} ms active {act:.3f}", flush=True)
     results.sort(key=lambda r: r[0])
-    return {"params": {k2: results[0][1][k2] for k2 in keys}, "ms": results[0][0],
-            "n": n, "candidates": len(results)}
+    ok = [r for r in results if r[1] >= min_active] or results
+    ms, act, best = ok[0]
+    return {"params": {k2: best[k2] for k2 in keys}, "ms": ms, "active_lane_fraction": act,
+            "min_active": min_active, "fastest_any_ms": results[0][0], "n": n,
+            "candidates": len(results)}
 
 
 BATCH_SPACE = {
     "live_threshold": [32, 48, 56],
-    "waves_per_cu": [0, 8, 12],
+    "waves_per_cu": [0, 12, 16],
+    "reconv": [0, 1],
 }
 
 
@@ -105,31 +114,32 @@ def tune_batch(mixed, reps: int = 2, space: dict | None = None, verbose: bool = 
             "threads": mixed.threads, "candidates": len(results)}
 
 
-def save(kernel_name: str, entry: dict) -> None:
-    data = json.loads(TUNED.read_text()) if TUNED.exists() else {}
+def save(kernel_name: str, entry: dict, path: Path | None = None) -> None:
+    path = path or TUNED
+    data = json.loads(path.read_text()) if path.exists() else {}
     data[kernel_name] = entry
-    TUNED.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+    path.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
 
 
 def main(argv=None) -> int:
+    import argparse
     from loom_amd import workloads
-    argv = argv or sys.argv[1:]
-    name = argv[0] if argv else "smallpt9_f32"
-    if name.startswith("batch:"):
-        from loom_amd import mixed
-        entry = tune_batch(mixed.MixedBatch())
-        save(name, entry)
+    ap = argparse.ArgumentParser()
+    ap.add_argument("names", nargs="*", default=["smallpt9_f32"])
+    ap.add_argument("--out", type=Path, default=TUNED, help="tuned settings file")
+    a = ap.parse_args(argv)
+    for name in a.names:
+        if name.startswith("batch:"):
+            from loom_amd import mixed
+            entry = tune_batch(mixed.MixedBatch())
+        elif name.startswith("smallpt9"):
+            entry = tune(name, workloads.pt_inputs(1024, 768, 64))  # the full frame
+        elif name.startswith("collatz"):
+            entry = tune(name, [workloads.collatz_inputs(1 << 24)])
+        else:
+            entry = tune(name, [workloads.fib_inputs(1 << 20)])
+        save(name, entry, a.out)
         print(json.dumps({name: entry}), flush=True)
-        return 0
-    if name.startswith("smallpt9"):
-        ins = workloads.pt_inputs(1024, 768, 8)       # a frame at 1/8 of the spp
-    elif name.startswith("collatz"):
-        ins = [workloads.collatz_inputs(1 << 22)]
-    else:
-        ins = [workloads.fib_inputs(1 << 20)]
-    entry = tune(name, ins)
-    save(name, entry)
-    print(json.dumps({name: entry}), flush=True)
     return 0
 
 
