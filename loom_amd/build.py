@@ -35,6 +35,10 @@ COMMON = [
     "-fPIC",
     "-fconstexpr-steps=200000000",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep the stores of different switch cases apart: merging them into one
+    # store with a computed register index would move a kernel's register
+    # file from VGPRs to scratch memory
+    "-mllvm", "-simplifycfg-sink-common=false",
     f"-I{ROOT / 'include' / 'loom_vm'}",
     f"-I{ROOT / 'include'}",
 ]

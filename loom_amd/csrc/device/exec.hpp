@@ -52,12 +52,47 @@ struct Layout
     static constexpr std::size_t n64 = map.n64 ? map.n64 : 1;
 };
 
+/// @brief N distinct scalar cells (one base class per cell).  Unlike an
+/// array, a cell can only be addressed with a compile-time index, so no
+/// optimisation (e.g. merging the stores of several switch cases into one
+/// store with a computed index) can force the register file out of VGPRs
+/// into scratch memory.
+template <std::size_t I, typename T>
+struct Cell
+{
+    T v;
+};
+
+template <typename T, typename Seq>
+struct CellsOf;
+
+template <typename T, std::size_t... I>
+struct CellsOf<T, std::index_sequence<I...>> : Cell<I, T>...
+{
+    template <std::size_t J>
+    LOOM_DEVICE T&
+    at()
+    {
+        return static_cast<Cell<J, T>&>(*this).v;
+    }
+
+    template <std::size_t J>
+    LOOM_DEVICE const T&
+    at() const
+    {
+        return static_cast<const Cell<J, T>&>(*this).v;
+    }
+};
+
+template <typename T, std::size_t N>
+using Cells = CellsOf<T, std::make_index_sequence<N>>;
+
 template <typename K>
 struct Regs
 {
     using L = Layout<K>;
-    std::uint32_t r32[L::n32];
-    std::uint64_t r64[L::n64];
+    Cells<std::uint32_t, L::n32> r32;
+    Cells<std::uint64_t, L::n64> r64;
 
     template <std::size_t slot, typename V>
     LOOM_DEVICE V
@@ -65,13 +100,13 @@ struct Regs
     {
         constexpr std::size_t i = L::map.idx[slot];
         if constexpr(L::T.reg[slot].width == 8)
-            return std::bit_cast<V>(r64[i]);
+            return std::bit_cast<V>(r64.template at<i>());
         else if constexpr(sizeof(V) == 4)
-            return std::bit_cast<V>(r32[i]);
+            return std::bit_cast<V>(r32.template at<i>());
         else if constexpr(sizeof(V) == 2)
-            return std::bit_cast<V>(static_cast<std::uint16_t>(r32[i]));
+            return std::bit_cast<V>(static_cast<std::uint16_t>(r32.template at<i>()));
         else
-            return std::bit_cast<V>(static_cast<std::uint8_t>(r32[i]));
+            return std::bit_cast<V>(static_cast<std::uint8_t>(r32.template at<i>()));
     }
 
     template <std::size_t slot, typename V>
@@ -80,13 +115,13 @@ struct Regs
     {
         constexpr std::size_t i = L::map.idx[slot];
         if constexpr(L::T.reg[slot].width == 8)
-            r64[i] = std::bit_cast<std::uint64_t>(v);
+            r64.template at<i>() = std::bit_cast<std::uint64_t>(v);
         else if constexpr(sizeof(V) == 4)
-            r32[i] = std::bit_cast<std::uint32_t>(v);
+            r32.template at<i>() = std::bit_cast<std::uint32_t>(v);
         else if constexpr(sizeof(V) == 2)
-            r32[i] = std::bit_cast<std::uint16_t>(v);
+            r32.template at<i>() = std::bit_cast<std::uint16_t>(v);
         else
-            r32[i] = std::bit_cast<std::uint8_t>(v);
+            r32.template at<i>() = std::bit_cast<std::uint8_t>(v);
     }
 
     /// @brief Raw slot bits (for spill/fill).
@@ -96,9 +131,9 @@ struct Regs
     {
         constexpr std::size_t i = L::map.idx[slot];
         if constexpr(L::T.reg[slot].width == 8)
-            return r64[i];
+            return r64.template at<i>();
         else
-            return r32[i];
+            return r32.template at<i>();
     }
 
     template <std::size_t slot>
@@ -107,9 +142,9 @@ struct Regs
     {
         constexpr std::size_t i = L::map.idx[slot];
         if constexpr(L::T.reg[slot].width == 8)
-            r64[i] = v;
+            r64.template at<i>() = v;
         else
-            r32[i] = static_cast<std::uint32_t>(v);
+            r32.template at<i>() = static_cast<std::uint32_t>(v);
     }
 };
 

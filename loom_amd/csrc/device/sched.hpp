@@ -145,14 +145,66 @@ uniform64(std::uint64_t v)
     return (std::uint64_t(hi) << 32) | lo;
 }
 
-/// @brief Per-wave scheduling state (wave-uniform values).
+/// @brief Per-wave scheduling state.  Counters of useful work are kept per
+/// lane (a VALU add under the block's EXEC mask, no popcount) and summed
+/// over the wave when flushed; the dispatch budget counts down.
 struct WaveCtx
 {
-    std::uint64_t act = 0, slots = 0, disp = 0, tsteps = 0, flops = 0;
-    std::uint64_t max_steps = 0;
-    std::uint32_t err = 0;   ///< lane-local error bits
-    bool          over = false;
+    std::uint64_t slots  = 0; ///< wave-uniform: sum of 64*ops(B) over dispatches
+    std::uint64_t budget = 0; ///< wave-uniform: dispatches left before the step budget trips
+    std::uint64_t issued = 0; ///< wave-uniform: dispatches already flushed
+    std::uint32_t lane_ops   = 0; ///< this lane: sum of ops(B) of the blocks it ran
+    std::uint32_t lane_steps = 0; ///< this lane: blocks it ran
+    std::uint32_t lane_flops = 0; ///< this lane: sum of flops(B) of the blocks it ran
+    std::uint32_t err        = 0; ///< this lane: error bits
+    bool          over       = false;
+
+    LOOM_DEVICE explicit WaveCtx(std::uint64_t max_steps) : budget(max_steps ? max_steps : 1) {}
 };
+
+/// @brief Step result: the next block (or rank) in the low bits, and
+/// kMayHalt when the block just run has HALT among its successors (only
+/// then can the number of live lanes have dropped).
+inline constexpr std::uint32_t kMayHalt = 1u << 16;
+inline constexpr std::uint32_t kIndex   = 0xFFFFu;
+
+/// @brief Run block @p B on the lanes at B and count the work.
+template <typename K, std::size_t B>
+LOOM_DEVICE void
+run_block(Regs<K>& r, std::uint32_t& pc, WaveCtx& w)
+{
+    constexpr ir::Block blk = ir::table_of<K>.block[B];
+    if(pc == B)
+    {
+        exec_block<K, B>(r, pc, w.err);
+        w.lane_ops += blk.ops;
+        w.lane_steps += 1;
+        if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
+    }
+    w.slots += 64ull * blk.ops;
+    w.budget -= 1;
+    if(w.budget == 0) w.over = true;
+}
+
+/// @brief True when HALT is a successor of block @p B.
+template <typename K, std::size_t B>
+inline constexpr bool may_halt = []
+{
+    constexpr ir::Block blk = ir::table_of<K>.block[B];
+    for(std::size_t j = 0; j < blk.ns; ++j)
+        if(ir::table_of<K>.succ[blk.s0 + j] == HALT) return true;
+    return blk.ns == 0;
+}();
+
+/// @brief Mask of the blocks that have HALT among their successors.
+template <typename K>
+inline constexpr std::uint64_t halt_blocks = []
+{
+    std::uint64_t m = 0;
+    [&]<std::size_t... B>(std::index_sequence<B...>)
+    { ((m |= may_halt<K, B> ? 1ull << B : 0ull), ...); }(std::make_index_sequence<ir::table_of<K>.nb>{});
+    return m;
+}();
 
 /// @brief Run block @p B on the lanes at B and pick the next block among its
 /// successors (threaded dispatch, simd.hpp:3577-3731).
@@ -160,16 +212,9 @@ template <typename K, std::size_t B>
 LOOM_DEVICE std::uint32_t
 step(Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, WaveCtx& w)
 {
-    constexpr ir::Block   blk = ir::table_of<K>.block[B];
+    constexpr ir::Block blk = ir::table_of<K>.block[B];
 
-    const std::uint64_t m = ballot(pc == B);
-    if(pc == B) exec_block<K, B>(r, pc, w.err);
-    const std::uint32_t n = popc(m);
-    w.act += std::uint64_t(n) * blk.ops;
-    w.slots += 64ull * blk.ops;
-    w.disp += 1;
-    w.tsteps += n;
-    w.flops += std::uint64_t(n) * blk.flops;
+    run_block<K, B>(r, pc, w);
     pending &= ~(1ull << B);
 
     if constexpr(blk.ns == 0)
@@ -301,13 +346,12 @@ run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t
     {
         while(cur != NONE)
         {
+            const std::uint32_t ran = cur;
             cur = dispatch<K>(cur, r, pc, pending, thr, w);
-            if(w.disp > w.max_steps)
-            {
-                w.over = true;
+            if(w.over) return;
+            if(live_thr && ((LOOM_CX(halt_blocks<K>) >> ran) & 1u) &&
+               popc(ballot(pc != HALT)) <= live_thr)
                 return;
-            }
-            if(live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
         }
         pops = 0;
         if(pending) cur = major_of_set(pending, pc, pops);
@@ -326,9 +370,36 @@ enum Policy : int
     POLICY_TOPO = 1,
 };
 
-/// @brief Mask of the blocks that hold live lanes.
+/// @brief Ranks of the topological policy: block order[R] has rank R, so
+/// the present set is kept as a mask of ranks and the next block is the
+/// lowest eligible rank (one find-first-set).
+template <typename K>
+struct TopoRanks
+{
+    static constexpr std::size_t nb = ir::table_of<K>.nb;
+
+    static constexpr auto rank = []
+    {
+        std::array<std::uint32_t, 64> a{};
+        for(std::size_t i = 0; i < nb; ++i) a[ir::table_of<K>.order[i]] = static_cast<std::uint32_t>(i);
+        return a;
+    }();
+
+    /// body masks of the loop heads, in rank space
+    static constexpr auto body = []
+    {
+        std::array<std::uint64_t, 64> a{};
+        for(std::size_t h = 0; h < ir::table_of<K>.nhead; ++h)
+            for(std::size_t b = 0; b < nb; ++b)
+                if((ir::table_of<K>.body[h] >> b) & 1u) a[h] |= 1ull << rank[b];
+        return a;
+    }();
+};
+
+/// @brief Rank mask of the blocks that hold live lanes.
+template <typename K>
 LOOM_DEVICE std::uint64_t
-present_blocks(std::uint32_t pc)
+present_ranks(std::uint32_t pc)
 {
     std::uint64_t live = ballot(pc != HALT), pres = 0;
     while(live)
@@ -336,12 +407,31 @@ present_blocks(std::uint32_t pc)
         const int           l = __builtin_ctzll(live);
         const std::uint32_t b = uniform(__builtin_amdgcn_readlane(pc, l));
         live &= ~ballot(pc == b);
-        pres |= 1ull << b;
+        switch(b)
+        {
+#define LOOM_RC(i)                                                                    \
+    case i:                                                                           \
+        if constexpr(i < ir::table_of<K>.nb) pres |= 1ull << LOOM_CX(TopoRanks<K>::rank[i]); \
+        break;
+            LOOM_RC(0) LOOM_RC(1) LOOM_RC(2) LOOM_RC(3) LOOM_RC(4) LOOM_RC(5) LOOM_RC(6)
+            LOOM_RC(7) LOOM_RC(8) LOOM_RC(9) LOOM_RC(10) LOOM_RC(11) LOOM_RC(12) LOOM_RC(13)
+            LOOM_RC(14) LOOM_RC(15) LOOM_RC(16) LOOM_RC(17) LOOM_RC(18) LOOM_RC(19)
+            LOOM_RC(20) LOOM_RC(21) LOOM_RC(22) LOOM_RC(23) LOOM_RC(24) LOOM_RC(25)
+            LOOM_RC(26) LOOM_RC(27) LOOM_RC(28) LOOM_RC(29) LOOM_RC(30) LOOM_RC(31)
+            LOOM_RC(32) LOOM_RC(33) LOOM_RC(34) LOOM_RC(35) LOOM_RC(36) LOOM_RC(37)
+            LOOM_RC(38) LOOM_RC(39) LOOM_RC(40) LOOM_RC(41) LOOM_RC(42) LOOM_RC(43)
+            LOOM_RC(44) LOOM_RC(45) LOOM_RC(46) LOOM_RC(47) LOOM_RC(48) LOOM_RC(49)
+            LOOM_RC(50) LOOM_RC(51) LOOM_RC(52) LOOM_RC(53) LOOM_RC(54) LOOM_RC(55)
+            LOOM_RC(56) LOOM_RC(57) LOOM_RC(58) LOOM_RC(59) LOOM_RC(60) LOOM_RC(61)
+            LOOM_RC(62) LOOM_RC(63)
+#undef LOOM_RC
+        default: break;
+        }
     }
     return pres;
 }
 
-/// @brief Next block under POLICY_TOPO among the blocks of @p present.
+/// @brief Next rank under POLICY_TOPO among the ranks of @p present.
 template <typename K>
 LOOM_DEVICE std::uint32_t
 topo_pick(std::uint64_t present)
@@ -350,39 +440,26 @@ topo_pick(std::uint64_t present)
     std::uint64_t elig = present;
     [&]<std::size_t... H>(std::index_sequence<H...>)
     {
-        ((elig &= (present & LOOM_CX(std::uint64_t(ir::table_of<K>.body[H])))
-                      ? ~(1ull << LOOM_CX(std::uint32_t(ir::table_of<K>.head[H])))
+        ((elig &= (present & LOOM_CX(TopoRanks<K>::body[H]))
+                      ? ~(1ull << LOOM_CX(TopoRanks<K>::rank[ir::table_of<K>.head[H]]))
                       : ~0ull),
          ...);
     }(std::make_index_sequence<ir::table_of<K>.nhead>{});
     if(!elig) elig = present;
-    std::uint32_t best = NONE;
-    [&]<std::size_t... R>(std::index_sequence<R...>)
-    {
-        (((elig >> LOOM_CX(std::uint32_t(ir::table_of<K>.order[R]))) & 1u
-              ? (best = LOOM_CX(std::uint32_t(ir::table_of<K>.order[R])), true)
-              : false) ||
-         ...);
-    }(std::make_index_sequence<ir::table_of<K>.nb>{});
-    return best;
+    return static_cast<std::uint32_t>(__builtin_ctzll(elig));
 }
 
-/// @brief POLICY_TOPO step: run block @p B, update the present set from its
-/// successors, pick the next block.
-template <typename K, std::size_t B>
+/// @brief POLICY_TOPO step: run the block of rank @p R, update the present
+/// set from its successors, pick the next rank.
+template <typename K, std::size_t R>
 LOOM_DEVICE std::uint32_t
 step_topo(Regs<K>& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
 {
-    constexpr ir::Block blk = ir::table_of<K>.block[B];
-    const std::uint64_t m   = ballot(pc == B);
-    if(pc == B) exec_block<K, B>(r, pc, w.err);
-    const std::uint32_t n = popc(m);
-    w.act += std::uint64_t(n) * blk.ops;
-    w.slots += 64ull * blk.ops;
-    w.disp += 1;
-    w.tsteps += n;
-    w.flops += std::uint64_t(n) * blk.flops;
-    present &= ~(1ull << B);
+    constexpr std::size_t   B   = ir::table_of<K>.order[R];
+    constexpr ir::Block     blk = ir::table_of<K>.block[B];
+    constexpr std::uint32_t mh  = may_halt<K, B> ? kMayHalt : 0u;
+    run_block<K, B>(r, pc, w);
+    present &= ~(1ull << R);
     [&]<std::size_t... J>(std::index_sequence<J...>)
     {
         (
@@ -390,11 +467,12 @@ step_topo(Regs<K>& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
             {
                 constexpr std::uint32_t s = ir::table_of<K>.succ[blk.s0 + J];
                 if constexpr(s != HALT)
-                    if(ballot(pc == s)) present |= 1ull << s;
+                    if(ballot(pc == s)) present |= 1ull << LOOM_CX(TopoRanks<K>::rank[s]);
             }(),
             ...);
     }(std::make_index_sequence<blk.ns>{});
-    return topo_pick<K>(present);
+    const std::uint32_t next = topo_pick<K>(present);
+    return next == NONE ? NONE : (next | mh);
 }
 
 #define LOOM_TCASE(b)                                                                     \
@@ -435,20 +513,16 @@ template <typename K>
 LOOM_DEVICE void
 run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
 {
-    std::uint64_t present = present_blocks(pc);
+    std::uint64_t present = present_ranks<K>(pc);
     std::uint32_t cur     = topo_pick<K>(present);
     if(cur == NONE) return;
     if(!force && live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
     for(;;)
     {
-        cur = dispatch_topo<K>(cur, r, pc, present, w);
-        if(w.disp > w.max_steps)
-        {
-            w.over = true;
-            return;
-        }
-        if(cur == NONE) return;
-        if(live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
+        const std::uint32_t res = dispatch_topo<K>(cur, r, pc, present, w);
+        if(w.over || res == NONE) return;
+        if(live_thr && (res & kMayHalt) && popc(ballot(pc != HALT)) <= live_thr) return;
+        cur = res & kIndex;
     }
 }
 
@@ -464,18 +538,46 @@ run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_
         run_to_threshold<K>(r, pc, thr, live_thr, force, w);
 }
 
-template <typename K>
-LOOM_DEVICE void
-flush_stats(const WaveCtx& w, const CtlArgs& c)
+/// @brief Sum of a per-lane u32 over the wave (every lane must be active).
+LOOM_DEVICE std::uint64_t
+wave_sum(std::uint32_t v)
 {
-    if(c.stats && lane_id() == 0)
+    std::uint64_t s = v;
+    for(int off = 32; off >= 1; off >>= 1)
+        s += static_cast<std::uint64_t>(__shfl_xor(static_cast<unsigned long long>(s), off, 64));
+    return uniform64(s);
+}
+
+/// @brief Add the wave's counters to the run's and clear them.  Called with
+/// every lane active: at the end of a kernel, and when a lane's counters
+/// approach their 32-bit range in long persistent runs.
+LOOM_DEVICE void
+flush_stats(WaveCtx& w, std::uint64_t max_steps, const CtlArgs& c)
+{
+    if(!c.stats) return;
+    const std::uint64_t act = wave_sum(w.lane_ops), steps = wave_sum(w.lane_steps),
+                        flops = wave_sum(w.lane_flops);
+    const std::uint64_t disp = (max_steps ? max_steps : 1) - w.budget - w.issued;
+    if(lane_id() == 0)
     {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_OPS]), w.act);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_OPS]), act);
         atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_SLOT_OPS]), w.slots);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_DISPATCHES]), w.disp);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_THREAD_STEPS]), w.tsteps);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_FLOPS]), w.flops);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_DISPATCHES]), disp);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_THREAD_STEPS]), steps);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_FLOPS]), flops);
     }
+    w.issued += disp;
+    w.slots      = 0;
+    w.lane_ops   = 0;
+    w.lane_steps = 0;
+    w.lane_flops = 0;
+}
+
+/// @brief Flush when some lane's counters passed half their range.
+LOOM_DEVICE void
+flush_if_high(WaveCtx& w, std::uint64_t max_steps, const CtlArgs& c)
+{
+    if(ballot(w.lane_ops >= 0x80000000u)) flush_stats(w, max_steps, c);
 }
 
 LOOM_DEVICE void
@@ -508,11 +610,10 @@ compact_kernel(IoArgs io, CtlArgs c)
         load_inputs<K>(r, io.in, tid);
         pc = 0;
     }
-    WaveCtx w;
-    w.max_steps = c.max_steps;
+    WaveCtx w(c.max_steps);
     run_policy<K, P>(r, pc, 0, 0, true, w);
     if(valid) store_outputs<K>(r, io.out, tid);
-    flush_stats<K>(w, c);
+    flush_stats(w, c.max_steps, c);
     raise_errors(w.err, w.over, c);
 }
 
@@ -756,12 +857,9 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 /// @brief One hybrid pass.  Launched with one wave per workgroup and a grid
 /// sized to the resident wave count; every wave loops claiming batches until
 /// the pass has none left, so each wave exits after finite work.  Waves
-/// beyond the pass's useful work exit at once.  The occupancy hint (at
-/// least 6 waves per SIMD, i.e. at most 80 VGPRs) fits the path tracer's
-/// topological form without spills and measured 2% faster than the 4 waves
-/// the register allocator picks on its own.
+/// beyond the pass's useful work exit at once.
 template <typename K, int P>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void
+__global__ __launch_bounds__(64) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
     HybridCtl* ctl            = h.ctl;
@@ -788,8 +886,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     char*          qout = h.q[par ^ 1u];
     std::uint32_t* qcnt = ctl->qcount[par ^ 1u];
 
-    WaveCtx w;
-    w.max_steps = c.max_steps;
+    WaveCtx w(c.max_steps);
     std::uint64_t spill_rows = 0, spill_bytes = 0, fill_rows = 0, fill_bytes = 0,
                   admitted = 0, refills = 0, batches = 0, spill_failed = 0;
     Reservoir res;
@@ -801,6 +898,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 
     for(;;)
     {
+        flush_if_high(w, c.max_steps, c);
         // ---- claim a batch (the wave is empty here) ----
         pc              = HALT;
         has             = false;
@@ -917,7 +1015,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         }
         if(w.over) break;
     }
-    flush_stats<K>(w, c);
+    flush_stats(w, c.max_steps, c);
     if(c.stats && lane == 0)
     {
         auto* st = reinterpret_cast<unsigned long long*>(c.stats);
