@@ -118,6 +118,24 @@ type_size(int t)
 }
 
 int
+validate_run(const KernelEntry& e, int mode, const void* const* inputs, void* const* outputs,
+             std::uint64_t n, const loom_params* params)
+{
+    if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
+        return fail(LOOM_E_ARG, "unknown mode %d", mode);
+    if(n > 0xFFFFFFFFull) return fail(LOOM_E_CAPACITY, "at most 2^32-1 threads per run");
+    if(n && (!inputs || !outputs)) return fail(LOOM_E_ARG, "null column array");
+    for(int k = 0; n && k < e.info.n_inputs; ++k)
+        if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
+    for(int k = 0; n && k < e.info.n_outputs; ++k)
+        if(!outputs[k]) return fail(LOOM_E_ARG, "output column %d is null", k);
+    if(params && (params->threshold > 63 || params->live_threshold > 63))
+        return fail(LOOM_E_ARG, "thresholds must be < 64");
+    if(params && params->reconv > 1) return fail(LOOM_E_ARG, "unknown reconvergence policy");
+    return 0;
+}
+
+int
 check_kernel(int k)
 {
     if(k < 0 || k >= static_cast<int>(registry().size()))
@@ -568,15 +586,7 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
     if(int r = ensure_device()) return r;
     if(int r = check_kernel(kernel)) return r;
     const KernelEntry& e = registry()[kernel];
-    if(n > 0xFFFFFFFFull) return fail(LOOM_E_CAPACITY, "at most 2^32-1 threads per run");
-    if(n && (!inputs || !outputs)) return fail(LOOM_E_ARG, "null column array");
-    for(int k = 0; n && k < e.info.n_inputs; ++k)
-        if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
-    for(int k = 0; n && k < e.info.n_outputs; ++k)
-        if(!outputs[k]) return fail(LOOM_E_ARG, "output column %d is null", k);
-    if(params && (params->threshold > 63 || params->live_threshold > 63))
-        return fail(LOOM_E_ARG, "thresholds must be < 64");
-    if(params && params->reconv > 1) return fail(LOOM_E_ARG, "unknown reconvergence policy");
+    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
     if(stats) *stats = loom_run_stats{};
     const IoArgs io = n ? make_io(e, inputs, outputs) : IoArgs{};
     CtlArgs      c;
@@ -609,6 +619,8 @@ loom_run_host(int kernel, int mode, const void* const* inputs, void* const* outp
     if(int r = ensure_device()) return r;
     if(int r = check_kernel(kernel)) return r;
     const KernelEntry& e  = registry()[kernel];
+    // validate before allocating or copying anything
+    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
     const int          ni = e.info.n_inputs, no = e.info.n_outputs;
     std::vector<void*> din(ni), dout(no);
     {
