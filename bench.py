@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("path-trace samples/sec + active-lane % (hybrid vs compact GPU); "
           "p99 req latency")
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published
+HBM_PEAK_GBPS = 8000.0    # MI355X HBM3E, published
 W, H, SPP = 1024, 768, 64
 SERVE_QPS = 1000.0
 SERVE_REQUESTS = 1000
@@ -45,6 +46,7 @@ def parse():
     ap.add_argument("--no-compact", action="store_true", help="skip the compact comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job mixed batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-regroup", action="store_true", help="skip the queue-regrouping form")
     ap.add_argument("--qps", type=float, default=SERVE_QPS, help="serving arrival rate")
     ap.add_argument("--requests", type=int, default=SERVE_REQUESTS)
     return ap.parse_args()
@@ -106,6 +108,34 @@ def serving(qps, count):
                  "workload": "Poisson arrivals; half frames 128^2-512^2 at 4-16 spp, half "
                              "fib/Collatz of 2^14-2^20 threads"})
     return summ
+
+
+def regroup_leg(k, ins, outs, h_img):
+    """The queue-regrouping form on the same frame: no in-wave refill, every
+    pass spills the survivors of each wave to the queue of their next label
+    (one atomic per wave and label, rows ranked with mbcnt) and the next pass
+    runs 64-row batches of one label.  Reports the regroup traffic (spilled
+    and filled row bytes) over the pass kernels' time."""
+    import loom_amd
+    from loom_amd import abi
+    out = {}
+    for live in (32, 48):
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, live_threshold=live, refill=False,
+                                 regroup=True)
+        loom_amd.run(k, abi.MODE_HYBRID, ins, outs, params=p)
+        r = loom_amd.run(k, abi.MODE_HYBRID, ins, outs, params=p)
+        same = all(np.array_equal(o.numpy().view(np.uint32), x.view(np.uint32))
+                   for o, x in zip(outs, h_img))
+        moved = r.counters[abi.C_SPILL_BYTES] + r.counters[abi.C_FILL_BYTES]
+        gbs = moved / (r.ms_main / 1e3) / 1e9
+        out[f"live{live}"] = {
+            "ms_total": r.ms_total, "ms_passes": r.ms_main, "passes": r.passes,
+            "samples_per_s": W * H * SPP / (r.ms_total / 1e3),
+            "active_lane_fraction": r.active_lane_fraction,
+            "spill_rows": r.counters[abi.C_SPILL_ROWS], "fill_rows": r.counters[abi.C_FILL_ROWS],
+            "regroup_bytes": moved, "regroup_GBps_over_pass_time": gbs,
+            "frac_of_hbm_peak": gbs / HBM_PEAK_GBPS, "outputs_identical": same}
+    return out
 
 
 def traffic_from_profile():
@@ -217,6 +247,8 @@ def main():
         "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
                 "(fma = 2) per launch over the launch time, against the FP32 vector peak"}
 
+    if not a.no_regroup:
+        result["regroup"] = regroup_leg(k, ins, outs, h_img)
     if not a.no_batch:
         result["batch"] = mixed_batch(max(1, min(a.steps, 3)))
     if not a.no_serve:
