@@ -30,6 +30,7 @@ METRIC = ("path-trace samples/sec + active-lane % (hybrid vs compact GPU); "
           "p99 req latency")
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published
 HBM_PEAK_GBPS = 8000.0    # MI355X HBM3E, published
+POLICY_NAMES = {0: "successor-major", 1: "topo", 2: "sweep"}
 W, H, SPP = 1024, 768, 64
 SERVE_QPS = 1000.0
 SERVE_REQUESTS = 1000
@@ -242,7 +243,7 @@ def main():
     result["roofline"] = {
         "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profile(),
-        "kernel": "hybrid_pass_kernel<smallpt9_f32, topo>",
+        "kernel": f"hybrid_pass_kernel<smallpt9_f32, {POLICY_NAMES[hyb['reconv']]}>",
         "launch_ms": launch_ms, "launches_per_step": launches,
         "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
                 "(fma = 2) per launch over the launch time, against the FP32 vector peak"}

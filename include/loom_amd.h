@@ -101,15 +101,17 @@ typedef struct loom_params
     uint32_t regroup;        /* hybrid: spill survivors to per-label queues */
     uint32_t reconv;         /* reconvergence policy: 0 = successor-major
                                 (reference simd.hpp:3974-4005), 1 = topological
-                                (loop heads wait for their bodies) */
+                                (loop heads wait for their bodies), 2 = sweep
+                                (every block in topological order under its
+                                mask, empty blocks skipped) */
     uint32_t waves_per_cu;   /* hybrid persistent grid; 0 = occupancy limit */
     uint32_t collect_stats;  /* fill loom_run_stats.counters */
     uint64_t queue_capacity; /* hybrid rows per label queue; 0 = automatic */
     uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40 */
 } loom_params;
 /* A NULL loom_params* means: threshold 0, refill on, regroup off, no stats,
- * successor-major for compact, topological with live_threshold 48 for
- * hybrid. */
+ * successor-major for compact, sweep with live_threshold 48 for hybrid and
+ * batched launches. */
 
 typedef struct loom_run_stats
 {

@@ -80,7 +80,7 @@ to_c(const params& p, int mode)
     c.refill         = p.refill;
     c.regroup        = p.regroup;
     c.reconv         = p.reconv >= 0 ? static_cast<std::uint32_t>(p.reconv) :
-                                       (mode == LOOM_MODE_HYBRID ? 1u : 0u);
+                                       (mode == LOOM_MODE_HYBRID ? 2u : 0u);
     c.waves_per_cu   = p.waves_per_cu;
     c.queue_capacity = p.queue_capacity;
     c.max_steps      = p.max_steps;

@@ -164,12 +164,12 @@ def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, wa
                 queue_capacity=0, max_steps=0, stats=True, reconv=None,
                 mode=abi.MODE_COMPACT) -> abi.Params:
     """Run parameters; `reconv` defaults to successor-major for the compact
-    form (the reference policy) and to the topological policy for hybrid,
+    form (the reference policy) and to the sweep policy for hybrid,
     `live_threshold` to 48 for hybrid (0 for compact, which does not
     refill)."""
     p = abi.Params()
     if reconv is None:
-        reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_TOPO
+        reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_SWEEP
     if live_threshold is None:
         live_threshold = 48 if mode == abi.MODE_HYBRID else 0
     p.reconv = int(reconv)

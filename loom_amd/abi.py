@@ -20,6 +20,7 @@ MODE_HYBRID = 1
 
 RECONV_SUCCESSOR_MAJOR = 0
 RECONV_TOPO = 1
+RECONV_SWEEP = 2
 
 # loom_counter
 C_ACTIVE_OPS, C_SLOT_OPS, C_DISPATCHES, C_THREAD_STEPS = 0, 1, 2, 3

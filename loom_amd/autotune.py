@@ -25,13 +25,13 @@ TUNED = Path(__file__).resolve().parent / "tuned.json"
 # the hybrid form is held to), trading a few percent of speed when needed
 MIN_ACTIVE = 0.70
 
-DEFAULT = {"live_threshold": 48, "regroup": False, "reconv": 1, "waves_per_cu": 0,
+DEFAULT = {"live_threshold": 48, "regroup": False, "reconv": 2, "waves_per_cu": 0,
            "threshold": 0, "refill": True}
 
 SPACE = {
     "live_threshold": [32, 40, 48, 56],
     "regroup": [False, True],
-    "reconv": [0, 1],
+    "reconv": [0, 1, 2],
     "waves_per_cu": [0, 16, 20],
 }
 
@@ -83,7 +83,7 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
 BATCH_SPACE = {
     "live_threshold": [32, 48, 56],
     "waves_per_cu": [0, 12, 16],
-    "reconv": [0, 1],
+    "reconv": [1, 2],
 }
 
 

@@ -67,7 +67,7 @@ claim_job_chunk(BatchCtl* ctl, const JobDesc* J, std::uint32_t j, std::uint32_t 
 
 /// @brief Run job @p j (kernel K) on this wave until the job has no more
 /// threads to admit and every lane halted.
-template <typename K>
+template <typename K, int P>
 LOOM_DEVICE void
 run_job(const BatchArgs& a, std::uint32_t j, const CtlArgs& c, WaveCtx& w, std::uint64_t& admitted)
 {
@@ -125,7 +125,7 @@ run_job(const BatchArgs& a, std::uint32_t j, const CtlArgs& c, WaveCtx& w, std::
             }
         }
         if(!ballot(has)) return;
-        run_policy<K, POLICY_TOPO>(r, pc, 0, exhausted ? 0u : c.live_thr, force, w);
+        run_policy<K, P>(r, pc, 0, exhausted ? 0u : c.live_thr, force, w);
         force = false;
         if(has && pc == HALT)
         {
@@ -142,8 +142,8 @@ struct BatchSet
     static constexpr std::uint32_t size = sizeof...(Ks);
 };
 
-/// @brief The persistent batch kernel over the kernels @p Ks.
-template <typename... Ks>
+/// @brief The persistent batch kernel over the kernels @p Ks (policy @p P).
+template <int P, typename... Ks>
 __global__ __launch_bounds__(64) void
 batch_kernel(CtlArgs c, BatchArgs a)
 {
@@ -164,7 +164,7 @@ batch_kernel(CtlArgs c, BatchArgs a)
         (
             [&]
             {
-                if(m == idx) run_job<Ks>(a, j, c, w, admitted);
+                if(m == idx) run_job<Ks, P>(a, j, c, w, admitted);
                 ++idx;
             }(),
             ...);

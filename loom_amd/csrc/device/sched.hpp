@@ -368,7 +368,15 @@ enum Policy : int
     /// lowest reverse-post-order block first; a loop header waits while any
     /// lane is inside its loop body (reconvergence at the loop head)
     POLICY_TOPO = 1,
+    /// predicated sweeps: every block in reverse post-order, each run under
+    /// the mask of its lanes and skipped when it has none; lanes that jump
+    /// forward run in the same sweep, lanes that jump back wait for the next
+    /// (so loop heads reconverge as under POLICY_TOPO) — no per-block choice,
+    /// popcount or present set, which suits kernels of small blocks
+    POLICY_SWEEP = 2,
 };
+
+inline constexpr int kNumPolicies = 3;
 
 /// @brief Ranks of the topological policy: block order[R] has rank R, so
 /// the present set is kept as a mask of ranks and the next block is the
@@ -526,7 +534,35 @@ run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, Wave
     }
 }
 
-/// @brief Run the wave under policy @p P (see run_to_threshold / run_topo).
+/// @brief POLICY_SWEEP runner: sweeps until every lane halted or, after a
+/// sweep, at most @p live_thr lanes are live (one sweep runs regardless when
+/// @p force is set).
+template <typename K>
+LOOM_DEVICE void
+run_sweep(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
+{
+    if(!ballot(pc != HALT)) return;
+    if(!force && live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
+    for(;;)
+    {
+        [&]<std::size_t... R>(std::index_sequence<R...>)
+        {
+            (
+                [&]
+                {
+                    constexpr std::size_t B = ir::table_of<K>.order[R];
+                    if(ballot(pc == B)) run_block<K, B>(r, pc, w);
+                }(),
+                ...);
+        }(std::make_index_sequence<ir::table_of<K>.nb>{});
+        if(w.over) return;
+        const std::uint64_t live = ballot(pc != HALT);
+        if(!live || (live_thr && popc(live) <= live_thr)) return;
+    }
+}
+
+/// @brief Run the wave under policy @p P (see run_to_threshold / run_topo /
+/// run_sweep).
 template <typename K, int P>
 LOOM_DEVICE void
 run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr, bool force,
@@ -534,6 +570,8 @@ run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_
 {
     if constexpr(P == POLICY_TOPO)
         run_topo<K>(r, pc, live_thr, force, w);
+    else if constexpr(P == POLICY_SWEEP)
+        run_sweep<K>(r, pc, live_thr, force, w);
     else
         run_to_threshold<K>(r, pc, thr, live_thr, force, w);
 }

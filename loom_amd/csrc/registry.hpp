@@ -22,12 +22,13 @@ namespace loom::hipvm
 struct KernelEntry
 {
     loom_kernel_info info{};
-    /// launchers per reconvergence policy (POLICY_SUCCESSOR_MAJOR, POLICY_TOPO)
-    hipError_t (*compact[2])(const IoArgs&, const CtlArgs&, hipStream_t) = {};
-    hipError_t (*hybrid_pass[2])(const IoArgs&, const CtlArgs&, const HybridArgs&,
+    /// launchers per reconvergence policy (POLICY_SUCCESSOR_MAJOR, POLICY_TOPO,
+    /// POLICY_SWEEP)
+    hipError_t (*compact[kNumPolicies])(const IoArgs&, const CtlArgs&, hipStream_t) = {};
+    hipError_t (*hybrid_pass[kNumPolicies])(const IoArgs&, const CtlArgs&, const HybridArgs&,
                                  unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
-    int (*hybrid_occupancy[2])()                                       = {};
+    int (*hybrid_occupancy[kNumPolicies])()                            = {};
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
     int (*kernel_attrs)(int which, int* vgprs, int* sgprs, int* scratch) = nullptr;
 };
@@ -41,26 +42,28 @@ struct BatchEntry
     char          name[64]{};
     std::uint32_t size = 0;
     std::uint64_t hash[kMaxJobs]{}; ///< structural hashes of the members
-    hipError_t (*launch)(const CtlArgs&, const BatchArgs&, unsigned grid, hipStream_t) = nullptr;
-    int (*occupancy)() = nullptr;
+    /// launchers and occupancy per reconvergence policy
+    hipError_t (*launch[kNumPolicies])(const CtlArgs&, const BatchArgs&, unsigned grid,
+                                       hipStream_t) = {};
+    int (*occupancy[kNumPolicies])() = {};
 };
 
 bool register_batch(const BatchEntry& e);
 
-template <typename... Ks>
+template <int P, typename... Ks>
 hipError_t
 launch_batch(const CtlArgs& c, const BatchArgs& a, unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_kernel<Ks...>), dim3(grid), dim3(64), 0, s, c, a);
+    hipLaunchKernelGGL((batch_kernel<P, Ks...>), dim3(grid), dim3(64), 0, s, c, a);
     return hipGetLastError();
 }
 
-template <typename... Ks>
+template <int P, typename... Ks>
 int
 batch_occupancy()
 {
     int nblk = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, batch_kernel<Ks...>, 64, 0) !=
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, batch_kernel<P, Ks...>, 64, 0) !=
        hipSuccess)
         return 1;
     return nblk > 0 ? nblk : 1;
@@ -75,8 +78,12 @@ make_batch(const char* name)
     e.size        = sizeof...(Ks);
     std::size_t i = 0;
     ((e.hash[i++] = ir::table_of<Ks>.hash), ...);
-    e.launch    = &launch_batch<Ks...>;
-    e.occupancy = &batch_occupancy<Ks...>;
+    e.launch[0]    = &launch_batch<POLICY_SUCCESSOR_MAJOR, Ks...>;
+    e.launch[1]    = &launch_batch<POLICY_TOPO, Ks...>;
+    e.launch[2]    = &launch_batch<POLICY_SWEEP, Ks...>;
+    e.occupancy[0] = &batch_occupancy<POLICY_SUCCESSOR_MAJOR, Ks...>;
+    e.occupancy[1] = &batch_occupancy<POLICY_TOPO, Ks...>;
+    e.occupancy[2] = &batch_occupancy<POLICY_SWEEP, Ks...>;
     return e;
 }
 
@@ -154,10 +161,13 @@ make_entry(const char* name)
     }
     e.compact[0]          = &launch_compact<K, POLICY_SUCCESSOR_MAJOR>;
     e.compact[1]          = &launch_compact<K, POLICY_TOPO>;
+    e.compact[2]          = &launch_compact<K, POLICY_SWEEP>;
     e.hybrid_pass[0]      = &launch_hybrid_pass<K, POLICY_SUCCESSOR_MAJOR>;
     e.hybrid_pass[1]      = &launch_hybrid_pass<K, POLICY_TOPO>;
+    e.hybrid_pass[2]      = &launch_hybrid_pass<K, POLICY_SWEEP>;
     e.hybrid_occupancy[0] = &hybrid_occupancy<K, POLICY_SUCCESSOR_MAJOR>;
     e.hybrid_occupancy[1] = &hybrid_occupancy<K, POLICY_TOPO>;
+    e.hybrid_occupancy[2] = &hybrid_occupancy<K, POLICY_SWEEP>;
     e.kernel_attrs     = &kernel_attrs<K>;
     return e;
 }

@@ -118,6 +118,16 @@ type_size(int t)
 }
 
 int
+validate_params(const loom_params* params)
+{
+    if(params && (params->threshold > 63 || params->live_threshold > 63))
+        return fail(LOOM_E_ARG, "thresholds must be < 64");
+    if(params && params->reconv >= static_cast<std::uint32_t>(kNumPolicies))
+        return fail(LOOM_E_ARG, "unknown reconvergence policy");
+    return 0;
+}
+
+int
 validate_run(const KernelEntry& e, int mode, const void* const* inputs, void* const* outputs,
              std::uint64_t n, const loom_params* params)
 {
@@ -129,10 +139,7 @@ validate_run(const KernelEntry& e, int mode, const void* const* inputs, void* co
         if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
     for(int k = 0; n && k < e.info.n_outputs; ++k)
         if(!outputs[k]) return fail(LOOM_E_ARG, "output column %d is null", k);
-    if(params && (params->threshold > 63 || params->live_threshold > 63))
-        return fail(LOOM_E_ARG, "thresholds must be < 64");
-    if(params && params->reconv > 1) return fail(LOOM_E_ARG, "unknown reconvergence policy");
-    return 0;
+    return validate_params(params);
 }
 
 int
@@ -197,7 +204,7 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
            loom_run_stats* st)
 {
     // persistent grid: resident 64-lane workgroups
-    const int policy = p ? static_cast<int>(p->reconv) : 1;
+    const int policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
     const int occ    = e.hybrid_occupancy[policy]();
     int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
     wpc           = std::max(1, std::min(wpc, occ));
@@ -375,10 +382,16 @@ make_ctl(const loom_params* p, std::uint64_t n)
     return c;
 }
 
+int
+batch_policy(const loom_params* p)
+{
+    return p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
+}
+
 unsigned
 batch_grid(const BatchEntry& b, const loom_params* p, std::uint64_t threads)
 {
-    const int occ = b.occupancy();
+    const int occ = b.occupancy[batch_policy(p)]();
     int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
     wpc           = std::max(1, std::min(wpc, occ));
     unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
@@ -696,6 +709,7 @@ loom_batch_run(int set, const loom_job* jobs, int njobs, const loom_params* para
 {
     std::lock_guard<std::mutex> lk(g_mutex);
     if(int r = ensure_device()) return r;
+    if(int r = validate_params(params)) return r;
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     if(stats) *stats = loom_run_stats{};
@@ -717,7 +731,7 @@ loom_batch_run(int set, const loom_job* jobs, int njobs, const loom_params* para
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
     LOOM_HIP(hipGetLastError());
     LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    LOOM_HIP(b.launch(c, a, grid, g_dev.stream));
+    LOOM_HIP(b.launch[batch_policy(params)](c, a, grid, g_dev.stream));
     LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
     LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
     float ms = 0;
@@ -755,6 +769,7 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     std::lock_guard<std::mutex> lk(g_mutex);
     if(int r = ensure_device()) return r;
     if(!graph) return fail(LOOM_E_ARG, "null graph");
+    if(int r = validate_params(params)) return r;
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     auto g = new loom_graph;
@@ -782,7 +797,7 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
                        g_dev.stats, static_cast<std::uint32_t>(kNumStats));
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
-    const hipError_t le = b.launch(c, a, g->grid, g_dev.stream);
+    const hipError_t le = b.launch[batch_policy(params)](c, a, g->grid, g_dev.stream);
     const hipError_t ce = hipStreamEndCapture(g_dev.stream, &g->graph);
     if(le != hipSuccess || ce != hipSuccess)
     {

@@ -48,12 +48,14 @@ def check_jobs(jobs):
             assert np.array_equal(bits(g), bits(w)), name
 
 
+@pytest.mark.parametrize("reconv", [abi.RECONV_SUCCESSOR_MAJOR, abi.RECONV_TOPO,
+                                    abi.RECONV_SWEEP])
 @pytest.mark.parametrize("live", [0, 32, 48, 60])
-def test_mixed_batch_matches_oracle(gpu, live):
+def test_mixed_batch_matches_oracle(gpu, live, reconv):
     b = loom_amd.Batch("pt_int_mix")
     jobs = build_mixed(b)
-    print(f"[gpu] batch pt_int_mix live={live} jobs={len(jobs)}", flush=True)
-    r = b.run(loom_amd.make_params(mode=abi.MODE_HYBRID, live_threshold=live,
+    print(f"[gpu] batch pt_int_mix live={live} reconv={reconv} jobs={len(jobs)}", flush=True)
+    r = b.run(loom_amd.make_params(mode=abi.MODE_HYBRID, live_threshold=live, reconv=reconv,
                                    max_steps=1 << 26))
     assert r.errors == 0
     check_jobs(jobs)

@@ -30,6 +30,12 @@ MODES = [
                                                 reconv=abi.RECONV_SUCCESSOR_MAJOR)),
     ("hybrid_topo_smallq", abi.MODE_HYBRID, dict(live_threshold=40, regroup=True,
                                                  queue_capacity=64, waves_per_cu=1)),
+    ("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+    ("hybrid_sweep_live48", abi.MODE_HYBRID, dict(live_threshold=48,
+                                                  reconv=abi.RECONV_SWEEP)),
+    ("hybrid_sweep_smallq", abi.MODE_HYBRID, dict(live_threshold=40, regroup=True,
+                                                  queue_capacity=64, waves_per_cu=1,
+                                                  reconv=abi.RECONV_SWEEP)),
 ]
 STEP_BUDGET = 1 << 24  # every kernel here finishes far below it
 
