@@ -33,7 +33,7 @@ struct JobDesc
 
 struct BatchCtl
 {
-    std::uint64_t next[kMaxJobs]; ///< claim cursor of each job
+    Slot64 next[kMaxJobs]; ///< claim cursor of each job, one per 256-byte block
 };
 
 struct BatchArgs
@@ -53,7 +53,7 @@ claim_job_chunk(BatchCtl* ctl, const JobDesc* J, std::uint32_t j, std::uint32_t 
     if(lane_id() == 0)
     {
         const std::uint64_t e = J[j].end;
-        const std::uint64_t k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next[j]),
+        const std::uint64_t k = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->next[j].v),
                                           static_cast<unsigned long long>(chunk));
         if(k < e)
         {
@@ -156,7 +156,7 @@ batch_kernel(CtlArgs c, BatchArgs a)
     {
         const std::uint32_t j = (blockIdx.x + step) % nj;
         // skip exhausted jobs cheaply
-        if(uniform64(__hip_atomic_load(&a.ctl->next[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >=
+        if(uniform64(__hip_atomic_load(&a.ctl->next[j].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >=
            a.jobs[j].end)
             continue;
         const std::uint32_t m = a.jobs[j].member;
@@ -171,7 +171,7 @@ batch_kernel(CtlArgs c, BatchArgs a)
     }
     flush_stats(w, c.max_steps, c);
     if(c.stats && lane_id() == 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ADMITTED]), admitted);
+        atomicAdd(&stat_slot(c)[ST_ADMITTED], static_cast<unsigned long long>(admitted));
     raise_errors(w.err, w.over, c);
 }
 
@@ -180,6 +180,6 @@ inline __global__ void
 batch_reset_kernel(BatchArgs a)
 {
     const std::uint32_t j = threadIdx.x;
-    if(j < a.njobs) a.ctl->next[j] = a.jobs[j].begin;
+    if(j < a.njobs) a.ctl->next[j].v = a.jobs[j].begin;
 }
 } // namespace loom::hipvm

@@ -36,6 +36,11 @@ namespace loom::hipvm
 {
 inline constexpr int         kMaxIO    = 16;
 inline constexpr std::size_t kNumStats = 13;
+/// Counter copies: a wave adds into copy (workgroup index mod kStatSlots),
+/// the host sums the copies, so hundreds of thousands of waves do not
+/// serialise their atomics on 13 addresses.
+inline constexpr std::size_t kStatSlots = 64;
+inline constexpr std::size_t kStatWords = kNumStats * kStatSlots;
 
 /// @brief Kernel input/output columns (device pointers, structure of arrays).
 struct IoArgs
@@ -72,7 +77,7 @@ struct CtlArgs
     std::uint32_t  refill    = 1;       ///< hybrid: refill free lanes in-wave
     std::uint32_t  regroup   = 1;       ///< hybrid: spill survivors to queues
     std::uint32_t* err       = nullptr; ///< error flags
-    std::uint64_t* stats     = nullptr; ///< kNumStats counters or null
+    std::uint64_t* stats     = nullptr; ///< kStatSlots copies of kNumStats counters, or null
 };
 
 /// @brief Input partitions of a hybrid run: one per XCD, so that waves
@@ -81,16 +86,25 @@ struct CtlArgs
 inline constexpr std::uint32_t kRanges = 8;
 
 /// @brief Hybrid control block, resident in device memory.
+/// @brief A 64-bit counter alone in its 256-byte block.  Device-scope
+/// atomics from every XCD meet at one memory-side unit per cache line, so
+/// counters that many waves hit must not share a line.
+struct alignas(256) Slot64
+{
+    std::uint64_t v;
+};
+
 struct HybridCtl
 {
-    std::uint64_t next[kRanges];  ///< next unadmitted thread of each range
-    std::uint64_t end[kRanges];   ///< end of each range
+    Slot64        next[kRanges];  ///< next unadmitted thread of each range
+    Slot64        end[kRanges];   ///< end of each range
     std::uint32_t batch_next;     ///< batch claim counter of the pass
     std::uint32_t nbatches;       ///< queue batches of the pass
     std::uint32_t parity;         ///< in-queue side of the pass
     std::uint32_t live;           ///< threads queued for the pass
     std::uint32_t passes;         ///< passes run so far
     std::uint32_t done;           ///< set when no thread remains
+    std::uint32_t exhausted;      ///< bit r: range r has no input left (read before claiming)
     std::uint32_t qcount[2][64];  ///< per side, per label queue fill
     std::uint32_t bprefix[65];    ///< batch prefix over labels of the pass
 };
@@ -576,6 +590,20 @@ run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_
         run_to_threshold<K>(r, pc, thr, live_thr, force, w);
 }
 
+/// @brief This workgroup's copy of the counters.
+LOOM_DEVICE unsigned long long*
+stat_slot(const CtlArgs& c)
+{
+    return reinterpret_cast<unsigned long long*>(c.stats + (blockIdx.x % kStatSlots) * kNumStats);
+}
+
+/// @brief Add @p v to counter @p i of a copy (no atomic for zero).
+LOOM_DEVICE void
+stat_add(unsigned long long* st, std::size_t i, std::uint64_t v)
+{
+    if(v) atomicAdd(&st[i], static_cast<unsigned long long>(v));
+}
+
 /// @brief Sum of a per-lane u32 over the wave (every lane must be active).
 LOOM_DEVICE std::uint64_t
 wave_sum(std::uint32_t v)
@@ -598,11 +626,12 @@ flush_stats(WaveCtx& w, std::uint64_t max_steps, const CtlArgs& c)
     const std::uint64_t disp = (max_steps ? max_steps : 1) - w.budget - w.issued;
     if(lane_id() == 0)
     {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_OPS]), act);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_SLOT_OPS]), w.slots);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_DISPATCHES]), disp);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_THREAD_STEPS]), steps);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&c.stats[ST_ACTIVE_FLOPS]), flops);
+        auto* st = stat_slot(c);
+        stat_add(st, ST_ACTIVE_OPS, act);
+        stat_add(st, ST_SLOT_OPS, w.slots);
+        stat_add(st, ST_DISPATCHES, disp);
+        stat_add(st, ST_THREAD_STEPS, steps);
+        stat_add(st, ST_ACTIVE_FLOPS, flops);
     }
     w.issued += disp;
     w.slots      = 0;
@@ -845,9 +874,19 @@ struct Reservoir
 {
     std::uint64_t base      = 0;
     std::uint32_t count     = 0;
-    std::uint32_t exhausted = 0;   ///< bit r: range r has no input left
-    std::uint32_t chunk     = 512; ///< next claim size
+    std::uint32_t exhausted = 0; ///< bit r: range r has no input left
+    std::uint32_t chunk     = 0; ///< next claim size (0: not sized yet)
 };
+
+/// @brief Guided claim size: about half a fair share of what a range still
+/// holds among the waves that draw from it, one batch at least, 512 at most.
+LOOM_DEVICE std::uint32_t
+guided_chunk(std::uint64_t left)
+{
+    const std::uint64_t share = gridDim.x / kRanges ? gridDim.x / kRanges : 1;
+    const std::uint64_t c     = left / (2 * share);
+    return static_cast<std::uint32_t>(c < 64 ? 64 : (c > 512 ? 512 : c));
+}
 
 /// @brief Take up to @p want consecutive thread indices (wave-uniform).
 LOOM_DEVICE std::uint32_t
@@ -856,17 +895,25 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 {
     if(res.count == 0)
     {
+        if(res.chunk == 0)
+        {
+            const std::uint64_t nx = ctl->next[home].v, en = ctl->end[home].v;
+            res.chunk              = guided_chunk(uniform64(nx < en ? en - nx : 0));
+        }
         std::uint64_t b = 0, e = 0;
         std::uint32_t got = 0, exh = res.exhausted;
         if(lane_id() == 0)
         {
+            // ranges another wave found empty: a plain load instead of one
+            // more serialised atomic per wave and range at the end of input
+            exh |= __hip_atomic_load(&ctl->exhausted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for(std::uint32_t i = 0; i < kRanges && !got; ++i)
             {
                 const std::uint32_t r = (home + i) % kRanges;
                 if((exh >> r) & 1u) continue;
-                e                     = ctl->end[r];
+                e                     = ctl->end[r].v;
                 const std::uint64_t k = atomicAdd(
-                    reinterpret_cast<unsigned long long*>(&ctl->next[r]),
+                    reinterpret_cast<unsigned long long*>(&ctl->next[r].v),
                     static_cast<unsigned long long>(res.chunk));
                 if(k < e)
                 {
@@ -874,16 +921,18 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
                     got = static_cast<std::uint32_t>(e - k < res.chunk ? e - k : res.chunk);
                 }
                 else
+                {
                     exh |= 1u << r;
+                    __hip_atomic_fetch_or(&ctl->exhausted, 1u << r, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
         res.base      = uniform64(b);
         res.count     = uniform(got);
         res.exhausted = uniform(exh);
-        // guided chunking: about 1/256 of what the range still holds
-        const std::uint64_t left = uniform64(e) - (res.base + res.count);
-        const std::uint64_t c    = left / 256;
-        res.chunk = static_cast<std::uint32_t>(c < 64 ? 64 : (c > 512 ? 512 : c));
+        // guided chunking on what the range still holds
+        res.chunk = guided_chunk(uniform64(e) - (res.base + res.count));
     }
     const std::uint32_t g = res.count < want ? res.count : want;
     base                  = res.base;
@@ -910,7 +959,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     std::uint64_t left = 0;
     for(std::uint32_t i = 0; i < kRanges; ++i)
     {
-        const std::uint64_t nx = ctl->next[i], e = ctl->end[i];
+        const std::uint64_t nx = ctl->next[i].v, e = ctl->end[i].v;
         left += nx < e ? (e - nx + 63) / 64 : 0;
     }
     left = uniform64(left);
@@ -1056,15 +1105,15 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     flush_stats(w, c.max_steps, c);
     if(c.stats && lane == 0)
     {
-        auto* st = reinterpret_cast<unsigned long long*>(c.stats);
-        atomicAdd(&st[ST_SPILL_ROWS], spill_rows);
-        atomicAdd(&st[ST_SPILL_BYTES], spill_bytes);
-        atomicAdd(&st[ST_FILL_ROWS], fill_rows);
-        atomicAdd(&st[ST_FILL_BYTES], fill_bytes);
-        atomicAdd(&st[ST_ADMITTED], admitted);
-        atomicAdd(&st[ST_REFILLS], refills);
-        atomicAdd(&st[ST_BATCHES], batches);
-        atomicAdd(&st[ST_SPILL_FAILED], spill_failed);
+        auto* st = stat_slot(c);
+        stat_add(st, ST_SPILL_ROWS, spill_rows);
+        stat_add(st, ST_SPILL_BYTES, spill_bytes);
+        stat_add(st, ST_FILL_ROWS, fill_rows);
+        stat_add(st, ST_FILL_BYTES, fill_bytes);
+        stat_add(st, ST_ADMITTED, admitted);
+        stat_add(st, ST_REFILLS, refills);
+        stat_add(st, ST_BATCHES, batches);
+        stat_add(st, ST_SPILL_FAILED, spill_failed);
     }
     raise_errors(w.err, w.over, c);
 }
@@ -1082,8 +1131,8 @@ hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
     {
         for(std::uint32_t r = 0; r < kRanges; ++r)
         {
-            ctl->next[r] = n * r / kRanges;
-            ctl->end[r]  = n * (r + 1) / kRanges;
+            ctl->next[r].v = n * r / kRanges;
+            ctl->end[r].v  = n * (r + 1) / kRanges;
         }
         ctl->batch_next = 0;
         ctl->nbatches   = 0;
@@ -1091,6 +1140,9 @@ hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
         ctl->live       = 0;
         ctl->passes     = 0;
         ctl->done       = n == 0;
+        ctl->exhausted  = 0;
+        for(std::uint32_t r = 0; r < kRanges; ++r)
+            if(ctl->next[r].v >= ctl->end[r].v) ctl->exhausted |= 1u << r;
         for(int side = 0; side < 2; ++side)
             for(int l = 0; l < 64; ++l) ctl->qcount[side][l] = 0;
         for(int l = 0; l < 65; ++l) ctl->bprefix[l] = 0;
@@ -1110,11 +1162,14 @@ hybrid_setup_kernel(HybridCtl* ctl, std::uint32_t nb, std::uint64_t qcap,
     ctl->bprefix[64] = pre;
     for(std::uint32_t l = 0; l < 64; ++l) ctl->qcount[out ^ 1u][l] = 0;
     bool input_left = false;
+    std::uint32_t exhausted = 0;
     for(std::uint32_t r = 0; r < kRanges; ++r)
     {
-        if(ctl->next[r] > ctl->end[r]) ctl->next[r] = ctl->end[r];
-        input_left |= ctl->next[r] < ctl->end[r];
+        if(ctl->next[r].v > ctl->end[r].v) ctl->next[r].v = ctl->end[r].v;
+        input_left |= ctl->next[r].v < ctl->end[r].v;
+        if(ctl->next[r].v >= ctl->end[r].v) exhausted |= 1u << r;
     }
+    ctl->exhausted  = exhausted;
     ctl->parity     = out;
     ctl->nbatches   = pre;
     ctl->batch_next = 0;

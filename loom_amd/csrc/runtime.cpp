@@ -165,14 +165,19 @@ finish_stats(loom_run_stats* st, const loom_params* p)
     LOOM_HIP(hipMemcpyAsync(g_dev.host_err, g_dev.err, sizeof(std::uint32_t),
                             hipMemcpyDeviceToHost, g_dev.stream));
     if(p && p->collect_stats)
-        LOOM_HIP(hipMemcpyAsync(g_dev.host_stats, g_dev.stats, sizeof(std::uint64_t) * kNumStats,
+        LOOM_HIP(hipMemcpyAsync(g_dev.host_stats, g_dev.stats, sizeof(std::uint64_t) * kStatWords,
                                 hipMemcpyDeviceToHost, g_dev.stream));
     LOOM_HIP(hipStreamSynchronize(g_dev.stream));
     if(st)
     {
         st->errors = *g_dev.host_err;
         if(p && p->collect_stats)
-            for(std::size_t i = 0; i < kNumStats; ++i) st->counters[i] = g_dev.host_stats[i];
+            for(std::size_t i = 0; i < kNumStats; ++i)
+            {
+                std::uint64_t sum = 0;
+                for(std::size_t k = 0; k < kStatSlots; ++k) sum += g_dev.host_stats[k * kNumStats + i];
+                st->counters[i] = sum;
+            }
     }
     if(*g_dev.host_err)
         return fail(LOOM_E_RUNTIME, "kernel raised error flags 0x%x", *g_dev.host_err);
@@ -324,9 +329,9 @@ launch_fill(void* dst, const void* pattern, std::uint64_t count)
 __global__ void
 zero_counters_kernel(std::uint32_t* err, std::uint64_t* stats, std::uint32_t nstats)
 {
-    const std::uint32_t i = threadIdx.x;
-    if(i == 0) *err = 0;
-    if(stats && i < nstats) stats[i] = 0;
+    if(threadIdx.x == 0) *err = 0;
+    if(stats)
+        for(std::uint32_t i = threadIdx.x; i < nstats; i += blockDim.x) stats[i] = 0;
 }
 
 int
@@ -443,9 +448,9 @@ loom_init(int device)
     g_dev.cus = prop.multiProcessorCount;
     LOOM_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     LOOM_HIP(hipMalloc(&g_dev.err, sizeof(std::uint32_t)));
-    LOOM_HIP(hipMalloc(&g_dev.stats, sizeof(std::uint64_t) * kNumStats));
+    LOOM_HIP(hipMalloc(&g_dev.stats, sizeof(std::uint64_t) * kStatWords));
     LOOM_HIP(hipHostMalloc(&g_dev.host_err, sizeof(std::uint32_t), hipHostMallocDefault));
-    LOOM_HIP(hipHostMalloc(&g_dev.host_stats, sizeof(std::uint64_t) * kNumStats, hipHostMallocDefault));
+    LOOM_HIP(hipHostMalloc(&g_dev.host_stats, sizeof(std::uint64_t) * kStatWords, hipHostMallocDefault));
     LOOM_HIP(hipHostMalloc(&g_dev.host_ctl, sizeof(HybridCtl), hipHostMallocDefault));
     for(auto& ev: g_dev.ev) LOOM_HIP(hipEventCreate(&ev));
     g_dev.ready = true;
@@ -613,7 +618,7 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
     c.stats     = params && params->collect_stats ? g_dev.stats : nullptr;
     LOOM_HIP(hipMemsetAsync(g_dev.err, 0, sizeof(std::uint32_t), g_dev.stream));
     if(c.stats)
-        LOOM_HIP(hipMemsetAsync(g_dev.stats, 0, sizeof(std::uint64_t) * kNumStats, g_dev.stream));
+        LOOM_HIP(hipMemsetAsync(g_dev.stats, 0, sizeof(std::uint64_t) * kStatWords, g_dev.stream));
     int r = 0;
     if(mode == LOOM_MODE_COMPACT)
         r = run_compact(e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
@@ -727,7 +732,7 @@ loom_batch_run(int set, const loom_job* jobs, int njobs, const loom_params* para
     a.njobs = static_cast<std::uint32_t>(njobs);
     const unsigned grid = batch_grid(b, params, threads);
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
-                       g_dev.stats, static_cast<std::uint32_t>(kNumStats));
+                       g_dev.stats, static_cast<std::uint32_t>(kStatWords));
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
     LOOM_HIP(hipGetLastError());
     LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
@@ -795,7 +800,7 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     // counters are cleared by a kernel node (no memset nodes in the graph)
     LOOM_HIP(hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
-                       g_dev.stats, static_cast<std::uint32_t>(kNumStats));
+                       g_dev.stats, static_cast<std::uint32_t>(kStatWords));
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
     const hipError_t le = b.launch[batch_policy(params)](c, a, g->grid, g_dev.stream);
     const hipError_t ce = hipStreamEndCapture(g_dev.stream, &g->graph);
