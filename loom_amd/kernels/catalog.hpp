@@ -5,6 +5,7 @@
 /// names.  X(name, kernel type).
 #pragma once
 
+#include "extras.hpp"
 #include "programs.hpp"
 #include "smallpt9.hpp"
 
@@ -17,7 +18,10 @@
     X("int_mix_u64", loom_amd::programs::int_mix<std::uint64_t>)      \
     X("int_mix_u32", loom_amd::programs::int_mix<std::uint32_t>)      \
     X("fp_mix_f32", loom_amd::programs::fp_mix<float>)                \
-    X("fp_mix_f64", loom_amd::programs::fp_mix<double>)
+    X("fp_mix_f64", loom_amd::programs::fp_mix<double>)               \
+    X("threefry2x32_u32", loom_amd::programs::threefry2x32<std::uint32_t>) \
+    X("cubic_bisect_f32", loom_amd::programs::cubic_bisect<float>)    \
+    X("cubic_bisect_f64", loom_amd::programs::cubic_bisect<double>)
 
 #define LOOM_AMD_CATALOG_PT(X) X("smallpt9_f32", loom_amd::smallpt::smallpt9<float>)
 

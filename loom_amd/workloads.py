@@ -100,3 +100,25 @@ def job_inputs(name: str, spec: dict):
     if name.startswith("fib"):
         return [fib_inputs(spec["n"], seed=spec["seed"])]
     return [collatz_inputs(spec["n"], seed=spec["seed"])]
+
+
+def uniform_real(seed: int, n: int, lo: float, hi: float, dtype=np.float64) -> np.ndarray:
+    """Reals uniform in [lo, hi) from the top 53 bits of splitmix64."""
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    return (lo + (hi - lo) * u).astype(dtype)
+
+
+def cubic_inputs(n: int, dtype=np.float32, seed: int = SEED + 6):
+    """Cubic coefficients as the reference's root-finding test draws them
+    (tests/vm_root_finding.cpp:73-78): a = ±U[1, B], b, c, d = U[-B, B] with
+    B = 100 for float32 and 1e5 for float64."""
+    bound = 100.0 if np.dtype(dtype) == np.float32 else 1e5
+    a = uniform_real(seed, n, 1.0, bound, dtype)
+    sign = np.where(splitmix64(seed + 1, n) & np.uint64(1), -1, 1).astype(dtype)
+    return [a * sign] + [uniform_real(seed + 2 + k, n, -bound, bound, dtype) for k in range(3)]
+
+
+def threefry_inputs(n: int, seed: int = SEED + 7):
+    """Counter words (c0, c1) for the Threefry kernel."""
+    r = splitmix64(seed, n)
+    return [(r & np.uint64(0xFFFFFFFF)).astype(np.uint32), (r >> np.uint64(32)).astype(np.uint32)]

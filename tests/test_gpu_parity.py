@@ -79,6 +79,9 @@ CASES = {
                              np.random.default_rng(21).uniform(-50, 50, n)],
     # path tracer: the first n samples of a 40x30 frame at 8 spp
     "smallpt9_f32": lambda n: [a[:n] for a in workloads.pt_inputs(40, 30, 8)],
+    "threefry2x32_u32": lambda n: workloads.threefry_inputs(n, seed=22),
+    "cubic_bisect_f32": lambda n: workloads.cubic_inputs(n, np.float32, seed=23),
+    "cubic_bisect_f64": lambda n: workloads.cubic_inputs(n, np.float64, seed=24),
 }
 
 

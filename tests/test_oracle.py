@@ -261,3 +261,103 @@ def test_smallpt9_image_invariants():
     assert img.max() <= 12.0 and 0.05 < img.mean() < 5.0
     # the ceiling light (top rows) is brighter than the floor (bottom rows)
     assert img[-3:].mean() > img[:3].mean()
+
+
+# ---- Threefry-2x32 and cubic bisection -------------------------------------------------
+def threefry_ref(c0, c1, k0=0xDEAD, k1=0xBEEF):
+    """Threefry-2x32, 32 rounds, from its published definition (Salmon et al.,
+    SC 2011): rotations 13 15 26 6 17 29 16 24, key words k0, k1 and
+    0x1BD11BDA ^ k0 ^ k1, an injection after every four rounds."""
+    M = np.uint64(0xFFFFFFFF)
+    ks = [np.uint64(k0), np.uint64(k1), np.uint64(0x1BD11BDA ^ k0 ^ k1)]
+    rot = [13, 15, 26, 6, 17, 29, 16, 24]
+    x0 = (c0.astype(np.uint64) + ks[0]) & M
+    x1 = (c1.astype(np.uint64) + ks[1]) & M
+    for r in range(32):
+        x0 = (x0 + x1) & M
+        s = np.uint64(rot[r % 8])
+        x1 = ((x1 << s) | (x1 >> (np.uint64(32) - s))) & M
+        x1 = x1 ^ x0
+        if r % 4 == 3:
+            j = r // 4 + 1
+            x0 = (x0 + ks[j % 3]) & M
+            x1 = (x1 + ks[(j + 1) % 3] + np.uint64(j)) & M
+    return x0.astype(np.uint32), x1.astype(np.uint32)
+
+
+def test_threefry_matches_definition():
+    c0, c1 = workloads.threefry_inputs(5000)
+    c0[:4] = [0, 0xFFFFFFFF, 0xFFFFFFFF - 39916800, 123]
+    c1[:4] = [0, 0xFFFFFFFF, 7, 0]
+    (x0, x1, n0, n1), err = O.run("threefry2x32_u32", [c0, c1])
+    assert err == 0
+    w0, w1 = threefry_ref(c0, c1)
+    assert np.array_equal(x0, w0) and np.array_equal(x1, w1)
+    ctr = (c1.astype(np.uint64) << np.uint64(32)) | c0.astype(np.uint64)
+    nxt = ctr + np.uint64(39916801)  # wraps modulo 2^64
+    assert np.array_equal(n0, (nxt & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    assert np.array_equal(n1, (nxt >> np.uint64(32)).astype(np.uint32))
+
+
+def _round32(q: Fraction) -> np.float32:
+    """q rounded to float32, nearest with ties to even (exact)."""
+    f = np.float32(float(q))
+    best = f
+    for cand in (np.nextafter(f, np.float32(-np.inf)), np.nextafter(f, np.float32(np.inf))):
+        dc, db = abs(Fraction(float(cand)) - q), abs(Fraction(float(best)) - q)
+        if dc < db or (dc == db and int(cand.view(np.uint32)) % 2 == 0):
+            best = cand
+    return np.float32(best)
+
+
+def _fma(dt, x, y, z):
+    q = Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z))
+    return _round32(q) if dt == np.float32 else np.float64(float(q))
+
+
+def bisect_ref(dt, a, b, c, d, err_lit):
+    """The bisection of the reference's validator (tests/root_finding.hpp:
+    193-216) in exact IEEE arithmetic of type dt, with the loop caps of the
+    kernel (40 widenings, 256 halvings)."""
+    a, b, c, d = (dt(v) for v in (a, b, c, d))
+
+    def cubic(x):
+        x2 = dt(x * x)
+        x3 = dt(x2 * x)
+        return _fma(dt, a, x3, _fma(dt, b, x2, _fma(dt, c, x, d)))
+
+    def split(p, q):
+        return (cubic(p) < 0) != (cubic(q) < 0)
+
+    lo, hi, w = dt(-10), dt(10), 0
+    while True:
+        s = split(lo, hi)
+        w += 1
+        if s or w >= 40:
+            break
+        lo, hi = dt(lo * dt(10)), dt(hi * dt(10))
+    mid = dt(dt(dt(hi - lo) * dt(0.5)) + lo)
+    fm, i = abs(cubic(mid)), 0
+    if fm >= err_lit:
+        while True:
+            left = split(lo, mid)
+            hi, lo = (mid, lo) if left else (hi, mid)
+            mid = dt(dt(dt(hi - lo) * dt(0.5)) + lo)
+            fm = abs(cubic(mid))
+            i += 1
+            if not (fm >= err_lit and i < 256):
+                break
+    return mid, i
+
+
+@pytest.mark.parametrize("name,dt", [("cubic_bisect_f32", np.float32),
+                                     ("cubic_bisect_f64", np.float64)])
+def test_cubic_bisect_exact(name, dt):
+    ins = workloads.cubic_inputs(120, dt)
+    (mid, it), err = O.run(name, ins)
+    assert err == 0
+    err_lit = dt(O.parse("f32" if dt == np.float32 else "f64", "0.1"))
+    for k in range(len(ins[0])):
+        m, i = bisect_ref(dt, *(col[k] for col in ins), err_lit)
+        assert mid[k].tobytes() == dt(m).tobytes() and int(it[k]) == i, (k, mid[k], m, it[k], i)
+        assert i < 256  # every input of the workload converges before the cap
