@@ -33,6 +33,7 @@ HBM_PEAK_GBPS = 8000.0    # MI355X HBM3E, published
 POLICY_NAMES = {0: "successor-major", 1: "topo", 2: "sweep"}
 W, H, SPP = 1024, 768, 64
 SERVE_QPS = 1000.0
+SERVE_QPS_HIGH = 3000.0
 SERVE_REQUESTS = 1000
 SERVE_BUDGET = 4 << 30
 SERVE_STEP = 4 << 20
@@ -257,6 +258,12 @@ def main():
         result["serving"] = sv
         result["p50_latency_ms"] = sv["p50_ms"]
         result["p99_latency_ms"] = sv["p99_ms"]
+        if a.qps == SERVE_QPS:
+            # a second load point, near saturation
+            hi = serving(SERVE_QPS_HIGH, a.requests)
+            result["serving_high_load"] = {k: hi[k] for k in (
+                "qps", "requests_per_s", "p50_ms", "p99_ms", "max_ms", "gpu_busy",
+                "threads_per_s", "peak_resident")}
 
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline()

@@ -152,6 +152,21 @@ int loom_synchronize(void);
  * reference applies when a function argument is a scalar instead of a range
  * (src/vm/bve.hpp:1125-1133).  A kernel, not a memset. */
 int loom_fill(void* dst, const void* pattern, size_t elem_bytes, uint64_t count);
+/* Background copies on a second stream (default priority) that overlap
+ * kernels on the main stream; copies complete in the order they were
+ * issued.  Each call returns a ticket: loom_copy_status reports whether the
+ * copy (and every earlier one) completed, loom_copy_wait blocks until it
+ * has.  The caller keeps both buffers alive until then; host buffers should
+ * be pinned (loom_host_alloc) for the copy to run asynchronously.  Replaces
+ * the CUDA test backend's staging copies (src/vm/cuda.hpp:1468-1578). */
+enum loom_copy_kind
+{
+    LOOM_COPY_H2D = 0,
+    LOOM_COPY_D2H = 1
+};
+int loom_copy_async(void* dst, const void* src, size_t bytes, int kind, uint64_t* ticket);
+int loom_copy_status(uint64_t ticket, int* done);
+int loom_copy_wait(uint64_t ticket);
 /* Pinned host memory for staging (the reference CUDA backend stages through
  * pinned buffers, src/vm/cuda.hpp:1041-1055). */
 int loom_host_alloc(void** ptr, size_t bytes);

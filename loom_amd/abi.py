@@ -29,6 +29,7 @@ C_ADMITTED, C_REFILLS, C_BATCHES, C_SPILL_FAILED = 8, 9, 10, 11
 C_ACTIVE_FLOPS = 12
 
 E_ARG, E_KERNEL, E_NODEVICE, E_RUNTIME, E_CAPACITY, E_STATE = range(1000, 1006)
+COPY_H2D, COPY_D2H = 0, 1
 
 ERR_ASSERT_FP = 1
 ERR_STEP_BUDGET = 2
@@ -149,6 +150,10 @@ SIGNATURES = {
     "loom_synchronize": (C.c_int, []),
     "loom_fill": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64]),
     "loom_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "loom_copy_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                  C.POINTER(C.c_uint64)]),
+    "loom_copy_status": (C.c_int, [C.c_uint64, C.POINTER(C.c_int)]),
+    "loom_copy_wait": (C.c_int, [C.c_uint64]),
     "loom_host_free": (C.c_int, [C.c_void_p]),
     "loom_run": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                            C.c_uint64, C.POINTER(Params), C.POINTER(RunStats)]),

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -73,6 +74,12 @@ struct Device
     std::vector<hipEvent_t> pass_ev;
     JobDesc*       jobs = nullptr; // device job table of direct batch runs
     BatchCtl*      bctl = nullptr;
+    // background copies: a second stream (default priority) and the events
+    // of outstanding tickets, oldest first (copies complete in order)
+    hipStream_t    copy{};
+    std::deque<std::pair<std::uint64_t, hipEvent_t>> tickets;
+    std::vector<hipEvent_t> spare_events;
+    std::uint64_t  last_ticket = 0;
 };
 
 Device g_dev;
@@ -447,6 +454,7 @@ loom_init(int device)
     g_dev.id  = device;
     g_dev.cus = prop.multiProcessorCount;
     LOOM_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    LOOM_HIP(hipStreamCreateWithFlags(&g_dev.copy, hipStreamNonBlocking));
     LOOM_HIP(hipMalloc(&g_dev.err, sizeof(std::uint32_t)));
     LOOM_HIP(hipMalloc(&g_dev.stats, sizeof(std::uint64_t) * kStatWords));
     LOOM_HIP(hipHostMalloc(&g_dev.host_err, sizeof(std::uint32_t), hipHostMallocDefault));
@@ -586,6 +594,69 @@ loom_host_free(void* ptr)
     if(int r = ensure_device()) return r;
     LOOM_HIP(hipHostFree(ptr));
     return 0;
+}
+
+int
+loom_copy_async(void* dst, const void* src, size_t bytes, int kind, uint64_t* ticket)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(!ticket) return fail(LOOM_E_ARG, "null ticket");
+    if(bytes && (!dst || !src)) return fail(LOOM_E_ARG, "null pointer");
+    if(kind != LOOM_COPY_H2D && kind != LOOM_COPY_D2H) return fail(LOOM_E_ARG, "unknown copy kind %d", kind);
+    if(bytes)
+        LOOM_HIP(hipMemcpyAsync(dst, src, bytes,
+                                kind == LOOM_COPY_H2D ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                                g_dev.copy));
+    hipEvent_t ev{};
+    if(!g_dev.spare_events.empty())
+    {
+        ev = g_dev.spare_events.back();
+        g_dev.spare_events.pop_back();
+    }
+    else
+        LOOM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    LOOM_HIP(hipEventRecord(ev, g_dev.copy));
+    *ticket = ++g_dev.last_ticket;
+    g_dev.tickets.emplace_back(*ticket, ev);
+    return 0;
+}
+
+int
+loom_copy_status(uint64_t ticket, int* done)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(!done) return fail(LOOM_E_ARG, "null result");
+    while(!g_dev.tickets.empty())
+    {
+        const hipError_t q = hipEventQuery(g_dev.tickets.front().second);
+        if(q == hipErrorNotReady) break;
+        if(q != hipSuccess) return fail(static_cast<int>(q), "copy: %s", hipGetErrorString(q));
+        g_dev.spare_events.push_back(g_dev.tickets.front().second);
+        g_dev.tickets.pop_front();
+    }
+    *done = g_dev.tickets.empty() || g_dev.tickets.front().first > ticket;
+    return 0;
+}
+
+int
+loom_copy_wait(uint64_t ticket)
+{
+    hipEvent_t ev{};
+    {
+        std::lock_guard<std::mutex> lk(g_mutex);
+        if(int r = ensure_device()) return r;
+        if(ticket > g_dev.last_ticket) return fail(LOOM_E_ARG, "ticket %llu was never issued",
+                                                   static_cast<unsigned long long>(ticket));
+        // outstanding tickets are consecutive: the one asked for, if still
+        // outstanding, is at (ticket - oldest) in the queue
+        if(!g_dev.tickets.empty() && g_dev.tickets.front().first <= ticket)
+            ev = g_dev.tickets[ticket - g_dev.tickets.front().first].second;
+    }
+    if(ev) LOOM_HIP(hipEventSynchronize(ev));
+    int done = 0;
+    return loom_copy_status(ticket, &done);
 }
 
 int

@@ -109,9 +109,10 @@ class Arena
 
 struct Resident
 {
-    std::uint64_t     req    = 0;
-    std::uint64_t     cursor = 0;
-    std::uint64_t     off    = 0, bytes = 0;
+    std::uint64_t      req    = 0;
+    std::uint64_t      cursor = 0;
+    std::uint64_t      off    = 0, bytes = 0;
+    std::uint64_t      ticket = 0; ///< last copy of this request (0: none)
     std::vector<char*> in, out;
 };
 } // namespace
@@ -194,8 +195,12 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         else if(reqs[i].n > 0xFFFFFFFFull) reqs[i].status = LOOM_E_CAPACITY;
     }
 
+    // A request moves through: pending (arrived, waiting for a slot and
+    // memory) -> loading (inputs copying in on the copy stream) -> running
+    // (in the steps) -> draining (outputs copying out) -> done.  Copies of
+    // one request overlap the steps of the others.
     std::deque<std::uint64_t> pending;
-    std::vector<Resident>     res;
+    std::vector<Resident>     loading, running, draining;
     std::uint64_t             next = 0, finished = 0;
 
     auto request_bytes = [&](const loom_request& q, std::vector<std::uint64_t>& cols)
@@ -214,24 +219,48 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         return b;
     };
 
-    auto complete = [&](Resident& r) -> int
+    // start copying the outputs of a request whose threads all ran
+    auto drain = [&](Resident& r) -> int
     {
         loom_request&           q  = reqs[r.req];
         const loom_kernel_info& ki = info.at(q.kernel);
+        r.ticket                   = 0;
         for(int k = 0; k < ki.n_outputs && q.n; ++k)
-            if(int e = loom_memcpy_d2h(q.outputs[k], r.out[k], q.n * type_bytes(ki.output_type[k])))
+            if(int e = loom_copy_async(q.outputs[k], r.out[k], q.n * type_bytes(ki.output_type[k]),
+                                       LOOM_COPY_D2H, &r.ticket))
                 return e;
-        s->arena.release(r.off, r.bytes);
-        q.done_ms = now();
-        st.threads += q.n;
-        ++finished;
+        draining.push_back(std::move(r));
         return 0;
+    };
+
+    auto ready = [&](const Resident& r, int& done) -> int
+    {
+        done = 1;
+        return r.ticket ? loom_copy_status(r.ticket, &done) : 0;
     };
 
     std::vector<std::uint64_t> cols;
     std::vector<loom_job>      jobs;
     while(finished < count)
     {
+        // ---- retire requests whose outputs arrived on the host ----
+        for(std::size_t i = 0; i < draining.size();)
+        {
+            int done = 0;
+            if(int e = ready(draining[i], done)) return e;
+            if(!done)
+            {
+                ++i;
+                continue;
+            }
+            loom_request& q = reqs[draining[i].req];
+            s->arena.release(draining[i].off, draining[i].bytes);
+            q.done_ms = now();
+            st.threads += q.n;
+            ++finished;
+            draining.erase(draining.begin() + static_cast<std::ptrdiff_t>(i));
+        }
+        // ---- arrivals ----
         const double t = now();
         while(next < count && (!realtime || reqs[next].arrival_ms <= t))
         {
@@ -241,8 +270,8 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
                 pending.push_back(next);
             ++next;
         }
-        // admission: FIFO, by slot and HBM budget
-        while(!pending.empty() && res.size() < s->cfg.max_resident)
+        // ---- admission: FIFO, by slot and HBM budget; inputs start copying ----
+        while(!pending.empty() && loading.size() + running.size() < s->cfg.max_resident)
         {
             loom_request&       q     = reqs[pending.front()];
             const std::uint64_t bytes = request_bytes(q, cols);
@@ -271,7 +300,8 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
                 const std::size_t tb = type_bytes(ki.input_type[k]);
                 const int         e  = (q.broadcast >> k) & 1u ?
                                            loom_fill(r.in[k], &q.scalars[k], tb, q.n) :
-                                           loom_memcpy_h2d(r.in[k], q.inputs[k], q.n * tb);
+                                           loom_copy_async(r.in[k], q.inputs[k], q.n * tb,
+                                                           LOOM_COPY_H2D, &r.ticket);
                 if(e) return e;
             }
             q.admit_ms = now();
@@ -279,15 +309,38 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             st.peak_bytes = std::max(st.peak_bytes, s->arena.used());
             if(q.n == 0)
             {
-                if(int e = complete(r)) return e;
+                if(int e = drain(r)) return e;
                 continue;
             }
-            res.push_back(std::move(r));
+            loading.push_back(std::move(r));
         }
-        st.peak_resident = std::max<std::uint32_t>(st.peak_resident, static_cast<std::uint32_t>(res.size()));
-        if(res.empty())
+        // ---- requests whose inputs arrived join the steps ----
+        for(std::size_t i = 0; i < loading.size();)
         {
-            if(next < count && realtime)
+            int done = 0;
+            if(int e = ready(loading[i], done)) return e;
+            if(!done)
+            {
+                ++i;
+                continue;
+            }
+            running.push_back(std::move(loading[i]));
+            loading.erase(loading.begin() + static_cast<std::ptrdiff_t>(i));
+        }
+        st.peak_resident = std::max<std::uint32_t>(
+            st.peak_resident, static_cast<std::uint32_t>(loading.size() + running.size()));
+        if(running.empty())
+        {
+            // nothing to run: wait for a copy, or idle until the next arrival
+            if(!loading.empty())
+            {
+                if(int e = loom_copy_wait(loading.front().ticket)) return e;
+            }
+            else if(!draining.empty())
+            {
+                if(int e = loom_copy_wait(draining.front().ticket)) return e;
+            }
+            else if(next < count && realtime)
             {
                 const double wait = reqs[next].arrival_ms - now();
                 if(wait > 0.05)
@@ -295,43 +348,43 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             }
             continue;
         }
-        // one step: a slice of every resident request in one batch launch
+        // ---- one step: a slice of every running request in one batch launch ----
         const std::uint64_t quota =
-            std::max<std::uint64_t>(64, (s->cfg.step_threads / res.size() + 63) / 64 * 64);
-        jobs.assign(res.size(), loom_job{});
-        for(std::size_t i = 0; i < res.size(); ++i)
+            std::max<std::uint64_t>(64, (s->cfg.step_threads / running.size() + 63) / 64 * 64);
+        jobs.assign(running.size(), loom_job{});
+        for(std::size_t i = 0; i < running.size(); ++i)
         {
-            const loom_request& q = reqs[res[i].req];
+            const loom_request& q = reqs[running[i].req];
             loom_job&           j = jobs[i];
             j.kernel              = q.kernel;
-            j.begin               = res[i].cursor;
-            j.end                 = std::min<std::uint64_t>(q.n, res[i].cursor + quota);
-            for(std::size_t k = 0; k < res[i].in.size(); ++k) j.inputs[k] = res[i].in[k];
-            for(std::size_t k = 0; k < res[i].out.size(); ++k) j.outputs[k] = res[i].out[k];
+            j.begin               = running[i].cursor;
+            j.end                 = std::min<std::uint64_t>(q.n, running[i].cursor + quota);
+            for(std::size_t k = 0; k < running[i].in.size(); ++k) j.inputs[k] = running[i].in[k];
+            for(std::size_t k = 0; k < running[i].out.size(); ++k) j.outputs[k] = running[i].out[k];
         }
         loom_run_stats rs{};
         if(int e = loom_batch_run(s->cfg.batch_set, jobs.data(), static_cast<int>(jobs.size()),
                                   &s->cfg.params, &rs))
         {
-            for(auto& r: res) reqs[r.req].status = e;
+            for(auto& r: running) reqs[r.req].status = e;
             return e;
         }
         st.gpu_ms += rs.ms_main;
         ++st.steps;
         std::vector<Resident> keep;
-        keep.reserve(res.size());
-        for(std::size_t i = 0; i < res.size(); ++i)
+        keep.reserve(running.size());
+        for(std::size_t i = 0; i < running.size(); ++i)
         {
-            res[i].cursor = jobs[i].end;
-            ++reqs[res[i].req].steps;
-            if(res[i].cursor >= reqs[res[i].req].n)
+            running[i].cursor = jobs[i].end;
+            ++reqs[running[i].req].steps;
+            if(running[i].cursor >= reqs[running[i].req].n)
             {
-                if(int e = complete(res[i])) return e;
+                if(int e = drain(running[i])) return e;
             }
             else
-                keep.push_back(std::move(res[i]));
+                keep.push_back(std::move(running[i]));
         }
-        res.swap(keep);
+        running.swap(keep);
     }
     st.makespan_ms = now();
     if(stats) *stats = st;
