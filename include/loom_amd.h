@@ -136,6 +136,14 @@ int         loom_device_info(int* compute_units, int* wave_size,
 int loom_kernel_count(void);
 int loom_kernel_info_get(int index, loom_kernel_info* out);
 int loom_kernel_find(const char* name);
+/* Text listing of the lowered kernel (blocks, cost, live-in registers,
+ * successors, instructions): the reference IR inspector's block view
+ * (src/common/dbg.hpp:1183-1202).  Writes at most len bytes including the
+ * terminating NUL; *needed receives the full size.  Needs no device. */
+int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
+/* VGPRs and scratch bytes per lane of a compiled kernel (mode: compact or
+ * hybrid pass kernel; policy: loom_params.reconv). */
+int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
 
 /* ---- device memory (the reference owns SoA columns in bve::Memory,
  * src/vm/bve.hpp:494-693; the CUDA backend in DeviceAllocator,

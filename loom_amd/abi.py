@@ -141,6 +141,9 @@ SIGNATURES = {
     "loom_kernel_count": (C.c_int, []),
     "loom_kernel_info_get": (C.c_int, [C.c_int, C.POINTER(KernelInfo)]),
     "loom_kernel_find": (C.c_int, [C.c_char_p]),
+    "loom_kernel_dump": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "loom_kernel_resources": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
     "loom_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "loom_free": (C.c_int, [C.c_void_p]),
     "loom_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -14,8 +14,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vm/dump.hpp>
+
 #include <cstdint>
 #include <cstring>
+#include <string>
 
 namespace loom::hipvm
 {
@@ -30,7 +33,8 @@ struct KernelEntry
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
     int (*hybrid_occupancy[kNumPolicies])()                            = {};
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
-    int (*kernel_attrs)(int which, int* vgprs, int* sgprs, int* scratch) = nullptr;
+    int (*kernel_attrs)(int mode, int policy, int* vgprs, int* scratch) = nullptr;
+    std::string (*dump)() = nullptr;
 };
 
 /// @brief Add @p e to the process-wide registry; returns true.
@@ -118,19 +122,36 @@ hybrid_occupancy()
     return nblk > 0 ? nblk : 1;
 }
 
+/// @brief Registers and scratch of the compiled kernel of @p mode and
+/// @p policy (LOOM_MODE_COMPACT: the predicated kernel; LOOM_MODE_HYBRID: the
+/// pass kernel).
 template <typename K>
 int
-kernel_attrs(int which, int* vgprs, int* sgprs, int* scratch)
+kernel_attrs(int mode, int policy, int* vgprs, int* scratch)
 {
+    const void* f = nullptr;
+    auto        pick = [&]<int P>()
+    {
+        f = mode == LOOM_MODE_COMPACT ? reinterpret_cast<const void*>(compact_kernel<K, P>) :
+                                        reinterpret_cast<const void*>(hybrid_pass_kernel<K, P>);
+    };
+    if(policy == POLICY_SUCCESSOR_MAJOR) pick.template operator()<POLICY_SUCCESSOR_MAJOR>();
+    else if(policy == POLICY_TOPO) pick.template operator()<POLICY_TOPO>();
+    else pick.template operator()<POLICY_SWEEP>();
     hipFuncAttributes a{};
-    const hipError_t  e =
-        which == 0 ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(compact_kernel<K, POLICY_TOPO>)) :
-                     hipFuncGetAttributes(&a, reinterpret_cast<const void*>(hybrid_pass_kernel<K, POLICY_TOPO>));
+    const hipError_t  e = hipFuncGetAttributes(&a, f);
     if(e != hipSuccess) return static_cast<int>(e);
     if(vgprs) *vgprs = a.numRegs;
-    if(sgprs) *sgprs = 0;
     if(scratch) *scratch = static_cast<int>(a.localSizeBytes);
     return 0;
+}
+
+/// @brief Text listing of the lowered table of @p K.
+template <typename K>
+std::string
+dump_of()
+{
+    return ir::dump_table(ir::table_of<K>);
 }
 
 template <typename K>
@@ -169,6 +190,7 @@ make_entry(const char* name)
     e.hybrid_occupancy[1] = &hybrid_occupancy<K, POLICY_TOPO>;
     e.hybrid_occupancy[2] = &hybrid_occupancy<K, POLICY_SWEEP>;
     e.kernel_attrs     = &kernel_attrs<K>;
+    e.dump             = &dump_of<K>;
     return e;
 }
 } // namespace loom::hipvm

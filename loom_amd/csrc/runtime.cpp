@@ -499,6 +499,34 @@ loom_kernel_info_get(int index, loom_kernel_info* out)
 }
 
 int
+loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed)
+{
+    if(int r = check_kernel(kernel)) return r;
+    const std::string text = registry()[kernel].dump();
+    if(needed) *needed = text.size() + 1;
+    if(buf && len)
+    {
+        const std::size_t n = std::min(len - 1, text.size());
+        std::memcpy(buf, text.data(), n);
+        buf[n] = 0;
+    }
+    return 0;
+}
+
+int
+loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes)
+{
+    if(int r = ensure_device()) return r;
+    if(int r = check_kernel(kernel)) return r;
+    if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
+        return fail(LOOM_E_ARG, "unknown mode %d", mode);
+    if(policy < 0 || policy >= kNumPolicies) return fail(LOOM_E_ARG, "unknown policy %d", policy);
+    if(int r = registry()[kernel].kernel_attrs(mode, policy, vgprs, scratch_bytes))
+        return fail(r, "hipFuncGetAttributes failed");
+    return 0;
+}
+
+int
 loom_kernel_find(const char* name)
 {
     if(!name) return -1;

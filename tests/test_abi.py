@@ -1,6 +1,7 @@
 """The C ABI library loads, exports every declared entry point, and reports
 kernel structure that matches hand derivations from the reference rules
 (core.hpp:686-1031).  No GPU call is made here."""
+import ctypes as C
 import re
 from pathlib import Path
 
@@ -69,3 +70,20 @@ def test_calls_before_init_fail_cleanly():
     p = C.c_void_p()
     assert lib.loom_malloc(C.byref(p), 16) == 1002
     assert b"loom_init" in lib.loom_last_error()
+
+
+def test_kernel_listing():
+    """The lowered listing of fib names its three blocks, their successors
+    and the loop's instructions (no device needed)."""
+    from loom_amd import inspect
+    text = inspect.listing("fib_u64")
+    assert "3 blocks" in text
+    assert "block 0 START" in text and "block 1 work" in text and "block 2 loop" in text
+    assert "successors: work HALT" in text and "successors: HALT loop" in text
+    assert "= add.u64 p c" in text and "pjump cmp -> HALT loop" in text
+    lib = abi.load()
+    need = C.c_size_t()
+    assert lib.loom_kernel_dump(999, None, 0, C.byref(need)) == abi.E_KERNEL
+    small = C.create_string_buffer(8)
+    assert lib.loom_kernel_dump(0, small, 8, C.byref(need)) == 0
+    assert len(small.value) == 7 and need.value > 8

@@ -1,0 +1,29 @@
+"""Compiled resources and the inspector's counter run.  Every kernel keeps
+its register file in VGPRs: no form or policy uses scratch memory (a
+regression guard for code shapes that demote the register file)."""
+import pytest
+
+import loom_amd
+from loom_amd import inspect
+
+pytestmark = pytest.mark.gpu
+
+
+def test_no_kernel_uses_scratch(gpu):
+    print("[gpu] kernel resources", flush=True)
+    for k in loom_amd.kernels():
+        res = inspect.resources(k.name)
+        for form, r in res.items():
+            assert 0 < r["vgprs"] <= 512, (k.name, form, r)
+            assert r["scratch_bytes"] == 0, (k.name, form, r)
+
+
+def test_inspector_counter_run(gpu):
+    print("[gpu] inspector counters", flush=True)
+    c = inspect.counters("collatz_u64", 4096)
+    for form in ("compact", "hybrid"):
+        assert c[form]["admitted"] in (0, 4096)
+        assert 0 < c[form]["active_lane_fraction"] <= 1
+        assert c[form]["active_ops"] <= c[form]["slot_ops"]
+    # the same threads run the same blocks in both forms
+    assert c["compact"]["thread_steps"] == c["hybrid"]["thread_steps"]
