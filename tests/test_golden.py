@@ -76,3 +76,19 @@ def test_literals_table():
 def test_literals_exact_where_representable(text, exact):
     row = next(r for r in load("literals.json")["literals"] if r["text"] == text)
     assert (row["ref_f32_bits"] == row["rn_f32_bits"]) == exact
+
+
+def test_ppm_of_fixture_frame(tmp_path):
+    """The image writer on the fixture frame: header, size, orientation (the
+    ceiling light is at the top of the file), deterministic bytes."""
+    from loom_amd import image
+    z = np.load(G / "smallpt9_32x24x8.npz")
+    img = z["image"]
+    path = tmp_path / "f.ppm"
+    image.write_ppm(path, img)
+    raw = path.read_bytes()
+    head = b"P6\n32 24\n255\n"
+    assert raw.startswith(head) and len(raw) == len(head) + 32 * 24 * 3
+    px = np.frombuffer(raw[len(head):], np.uint8).reshape(24, 32, 3)
+    assert px[:3].mean() > px[-3:].mean()  # light near the top row of the file
+    assert np.array_equal(px, image.to_bytes(img[::-1]))

@@ -28,3 +28,19 @@ def test_facade_builds(tmp_path, monkeypatch):
     if not build.LIB.exists():
         pytest.skip("library not built")
     assert build.build_facade_test().exists()
+
+
+@pytest.mark.gpu
+def test_render_and_write_ppm(gpu, tmp_path):
+    """End to end: a frame rendered by the hybrid form equals the oracle's
+    image, and the PPM writer produces the file."""
+    import numpy as np
+    from loom_amd import image, workloads
+    from tests import oracle_lib as O
+    w, h, spp = 48, 32, 4
+    print("[gpu] render ppm", flush=True)
+    img = image.render(w, h, spp)
+    want = workloads.pt_image(*O.run("smallpt9_f32", workloads.pt_inputs(w, h, spp))[0], w, h, spp)
+    assert np.array_equal(img, want)
+    image.write_ppm(tmp_path / "frame.ppm", img)
+    assert (tmp_path / "frame.ppm").stat().st_size == len(b"P6\n48 32\n255\n") + w * h * 3
