@@ -138,12 +138,41 @@ int loom_kernel_info_get(int index, loom_kernel_info* out);
 int loom_kernel_find(const char* name);
 /* Text listing of the lowered kernel (blocks, cost, live-in registers,
  * successors, instructions): the reference IR inspector's block view
- * (src/common/dbg.hpp:1183-1202).  Writes at most len bytes including the
+ * (src/vm/dbg.hpp:1183-1202).  Writes at most len bytes including the
  * terminating NUL; *needed receives the full size.  Needs no device. */
 int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
 /* VGPRs and scratch bytes per lane of a compiled kernel (mode: compact or
  * hybrid pass kernel; policy: loom_params.reconv). */
 int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
+/* Register r (0 <= r < n_registers) of a kernel: its name (NUL-terminated,
+ * at most len bytes), first type (loom_type), width in bytes, number of
+ * types (> 1: a union register, one name used with several same-size types,
+ * src/vm/core.hpp:993-1003) and LOOM_REG_* flags — the register list of the
+ * reference inspector's info view (src/vm/dbg.hpp:1183-1200).  Any output
+ * pointer may be NULL.  Needs no device. */
+#define LOOM_REG_INPUT   1 /* a kernel input */
+#define LOOM_REG_OUTPUT  2 /* a kernel output */
+#define LOOM_REG_WRITTEN 4 /* the destination of some instruction */
+int loom_kernel_register(int kernel, int r, char* name, size_t len, int* type, int* width,
+                         int* ntypes, int* flags);
+/* Per-thread step trace, the GPU counterpart of the reference's
+ * step-by-step inspector (src/vm/dbg.hpp:1011-1140; one row per block as
+ * print_state, 920-935).  Threads [0, n) of the device input columns run one
+ * block at a time, at most max_steps blocks each, with the same lowered block
+ * code as the compact and hybrid forms.  After every block one row of
+ * LOOM_TRACE_WORDS(n_registers) 64-bit words is written to `rows` (device):
+ *   word 0      block run (low 32 bits) | next block (high 32 bits,
+ *               LOOM_TRACE_HALT when the thread halted)
+ *   word 1      error bits so far (LOOM_ERR_ASSERT_FP)
+ *   word 2 + r  bits of register r, zero-extended; 0 before its first write.
+ * Thread t's rows start at row t * max_steps; steps[t] (device) receives the
+ * number of rows written for t (fewer than max_steps only if t halted).
+ * rows_bytes must hold n * max_steps rows; n < 2^20, 1 <= max_steps <= 2^16.
+ * Returns when the rows are written. */
+#define LOOM_TRACE_WORDS(n_registers) (2 + (n_registers))
+#define LOOM_TRACE_HALT 0xFFFFu
+int loom_kernel_trace(int kernel, const void* const* inputs, uint64_t n, uint32_t max_steps,
+                      uint64_t* rows, size_t rows_bytes, uint32_t* steps);
 
 /* ---- device memory (the reference owns SoA columns in bve::Memory,
  * src/vm/bve.hpp:494-693; the CUDA backend in DeviceAllocator,

@@ -34,6 +34,14 @@ COPY_H2D, COPY_D2H = 0, 1
 ERR_ASSERT_FP = 1
 ERR_STEP_BUDGET = 2
 
+REG_INPUT, REG_OUTPUT, REG_WRITTEN = 1, 2, 4  # loom_kernel_register flags
+TRACE_HALT = 0xFFFF  # next-block value of a trace row after the last block
+
+
+def trace_words(n_registers: int) -> int:
+    """64-bit words per step-trace row (LOOM_TRACE_WORDS)."""
+    return 2 + n_registers
+
 
 class KernelInfo(C.Structure):
     _fields_ = [
@@ -144,6 +152,11 @@ SIGNATURES = {
     "loom_kernel_dump": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "loom_kernel_resources": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
+    "loom_kernel_register": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "loom_kernel_trace": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint32,
+                                    C.c_void_p, C.c_size_t, C.c_void_p]),
     "loom_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "loom_free": (C.c_int, [C.c_void_p]),
     "loom_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

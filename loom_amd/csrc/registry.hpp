@@ -9,6 +9,7 @@
 #pragma once
 
 #include "device/batch.hpp"
+#include "device/trace.hpp"
 
 #include <loom_amd.h>
 
@@ -35,6 +36,13 @@ struct KernelEntry
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
     int (*kernel_attrs)(int mode, int policy, int* vgprs, int* scratch) = nullptr;
     std::string (*dump)() = nullptr;
+    /// step trace of threads [0, n) (trace_kernel)
+    hipError_t (*trace)(const IoArgs&, std::uint64_t n, std::uint32_t max_steps,
+                        std::uint64_t* rows, std::uint32_t* steps, hipStream_t) = nullptr;
+    /// name, first type, width, type count and LOOM_REG_* flags of register r
+    /// (returns 0 when r < nr)
+    int (*reg)(int r, const char** name, std::size_t* name_len, int* type, int* width,
+               int* ntypes, int* flags) = nullptr;
 };
 
 /// @brief Add @p e to the process-wide registry; returns true.
@@ -146,6 +154,37 @@ kernel_attrs(int mode, int policy, int* vgprs, int* scratch)
     return 0;
 }
 
+template <typename K>
+hipError_t
+launch_trace(const IoArgs& io, std::uint64_t n, std::uint32_t max_steps, std::uint64_t* rows,
+             std::uint32_t* steps, hipStream_t s)
+{
+    if(n == 0) return hipSuccess;
+    const std::uint64_t blocks = (n + 63) / 64;
+    hipLaunchKernelGGL((trace_kernel<K>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, io,
+                       n, max_steps, rows, steps);
+    return hipGetLastError();
+}
+
+template <typename K>
+int
+reg_of(int r, const char** name, std::size_t* name_len, int* type, int* width, int* ntypes,
+       int* flags)
+{
+    constexpr const auto& T = ir::table_of<K>;
+    if(r < 0 || static_cast<std::size_t>(r) >= T.nr) return 1;
+    const ir::Reg& g = T.reg[r];
+    if(name) *name = g.name.data();
+    if(name_len) *name_len = g.name.size();
+    if(type) *type = static_cast<int>(g.ty);
+    if(width) *width = g.width;
+    if(ntypes) *ntypes = g.ntypes;
+    if(flags)
+        *flags = (g.input ? LOOM_REG_INPUT : 0) | (g.output ? LOOM_REG_OUTPUT : 0) |
+                 (g.written ? LOOM_REG_WRITTEN : 0);
+    return 0;
+}
+
 /// @brief Text listing of the lowered table of @p K.
 template <typename K>
 std::string
@@ -191,6 +230,8 @@ make_entry(const char* name)
     e.hybrid_occupancy[2] = &hybrid_occupancy<K, POLICY_SWEEP>;
     e.kernel_attrs     = &kernel_attrs<K>;
     e.dump             = &dump_of<K>;
+    e.trace            = &launch_trace<K>;
+    e.reg              = &reg_of<K>;
     return e;
 }
 } // namespace loom::hipvm

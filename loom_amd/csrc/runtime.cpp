@@ -527,6 +527,52 @@ loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch
 }
 
 int
+loom_kernel_register(int kernel, int r, char* name, size_t len, int* type, int* width, int* ntypes,
+                     int* flags)
+{
+    if(int rc = check_kernel(kernel)) return rc;
+    const char* nm = nullptr;
+    std::size_t nl = 0;
+    if(registry()[kernel].reg(r, &nm, &nl, type, width, ntypes, flags))
+        return fail(LOOM_E_ARG, "kernel %s has no register %d", registry()[kernel].info.name, r);
+    if(name && len)
+    {
+        const std::size_t n = std::min(len - 1, nl);
+        std::memcpy(name, nm, n);
+        name[n] = 0;
+    }
+    return 0;
+}
+
+int
+loom_kernel_trace(int kernel, const void* const* inputs, uint64_t n, uint32_t max_steps,
+                  uint64_t* rows, size_t rows_bytes, uint32_t* steps)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if(int r = ensure_device()) return r;
+    if(int r = check_kernel(kernel)) return r;
+    const KernelEntry& e = registry()[kernel];
+    if(n >= (1ull << 20)) return fail(LOOM_E_CAPACITY, "at most 2^20-1 traced threads");
+    if(max_steps == 0 || max_steps > (1u << 16))
+        return fail(LOOM_E_ARG, "max_steps must be in [1, 65536]");
+    if(n == 0) return 0;
+    if(!inputs || !rows || !steps) return fail(LOOM_E_ARG, "null pointer argument");
+    for(int k = 0; k < e.info.n_inputs; ++k)
+        if(!inputs[k]) return fail(LOOM_E_ARG, "input column %d is null", k);
+    // n < 2^20, max_steps <= 2^16, words <= 2 + max_regs: the product fits in 64 bits
+    const std::uint64_t need =
+        n * max_steps * std::uint64_t(LOOM_TRACE_WORDS(e.info.n_registers)) * sizeof(std::uint64_t);
+    if(rows_bytes < need)
+        return fail(LOOM_E_CAPACITY, "trace rows need %llu bytes, %zu given",
+                    static_cast<unsigned long long>(need), rows_bytes);
+    IoArgs io{};
+    for(int k = 0; k < e.info.n_inputs; ++k) io.in[k] = inputs[k];
+    LOOM_HIP(e.trace(io, n, max_steps, rows, steps, g_dev.stream));
+    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    return 0;
+}
+
+int
 loom_kernel_find(const char* name)
 {
     if(!name) return -1;

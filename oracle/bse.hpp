@@ -736,16 +736,22 @@ eval_compute(const Stmt& s, std::uint64_t* R, std::uint32_t& err)
     }
 }
 
-/// Run one thread to HALT.  R holds one 64-bit word per register (low bytes
-/// significant), zeroed by the caller except for the inputs.  Returns the
-/// number of executed statements (labels excluded).
-inline std::uint64_t
-run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
-           std::uint64_t max_statements = ~std::uint64_t(0))
+/// Run one thread to HALT, calling on_block(first) after every block with the
+/// index of the block's first statement; on_block returns false to stop.
+/// A block ends where control reaches a label or the end of the code (a
+/// pending jump is taken there, else control falls through).  R holds one
+/// 64-bit word per register (low bytes significant), zeroed by the caller
+/// except for the inputs.  Returns the number of executed statements
+/// (labels excluded).
+template <typename OnBlock>
+std::uint64_t
+run_blocks(const Program& p, std::uint64_t* R, std::uint32_t& err, std::uint64_t max_statements,
+           OnBlock&& on_block)
 {
     const std::size_t n       = p.code.size();
     std::size_t       ip      = p.first;
     int               pending = -3; // -3: none
+    long              first   = -1; // first statement of the current block
     std::uint64_t     count   = 0;
     auto              go      = [&](int t) -> bool
     {
@@ -753,10 +759,18 @@ run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
         ip = t == -2 ? p.first : static_cast<std::size_t>(t);
         return true;
     };
+    auto end_block = [&]() -> bool
+    {
+        if(first < 0) return true;
+        const long f = first;
+        first        = -1;
+        return on_block(static_cast<std::size_t>(f));
+    };
     for(;;)
     {
         if(ip >= n)
         {
+            if(!end_block()) return count;
             if(pending != -3 && go(pending))
             {
                 pending = -3;
@@ -767,6 +781,7 @@ run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
         const Stmt& s = p.code[ip];
         if(s.kind == Stmt::label)
         {
+            if(!end_block()) return count;
             if(pending != -3)
             {
                 const int t = pending;
@@ -782,6 +797,7 @@ run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
             err |= 2u;
             return count;
         }
+        if(first < 0) first = static_cast<long>(ip);
         if(s.kind == Stmt::compute)
         {
             eval_compute(s, R, err);
@@ -802,5 +818,24 @@ run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
         }
         ++ip;
     }
+}
+
+/// Run one thread to HALT (run_blocks without a per-block hook).
+inline std::uint64_t
+run_thread(const Program& p, std::uint64_t* R, std::uint32_t& err,
+           std::uint64_t max_statements = ~std::uint64_t(0))
+{
+    return run_blocks(p, R, err, max_statements, [](std::size_t) { return true; });
+}
+
+/// Names of the labels directly before statement @p i (the labels that name
+/// the block starting there), "START" first when @p i is the first statement.
+inline std::string
+labels_before(const Program& p, std::size_t i)
+{
+    std::string out = i == p.first ? "START" : "";
+    for(std::size_t j = i; j > 0 && p.code[j - 1].kind == Stmt::label; --j)
+        for(const auto& l: p.code[j - 1].labels) out += (out.empty() ? "" : " ") + l;
+    return out;
 }
 } // namespace loom_oracle

@@ -137,6 +137,66 @@ oracle_run(int k, const void* const* in, void* const* out, uint64_t begin, uint6
     return 0;
 }
 
+/// Step trace of thread t of host columns: after every block, first[s] = the
+/// index of the block's first statement, errs[s] = error bits so far and
+/// regs[s * nregs + r] = bits of register r (oracle register order).
+int
+oracle_trace(int k, const void* const* in, uint64_t t, uint32_t max_steps, int32_t* first,
+             uint64_t* regs, uint32_t* errs, uint32_t* steps)
+{
+    if(k < 0 || k >= g_count || !in || !first || !regs || !errs || !steps) return 1;
+    const Program&             p = programs()[k];
+    const std::size_t          nr = p.reg_name.size();
+    std::vector<std::uint64_t> R(nr, 0);
+    for(std::size_t j = 0; j < p.in_reg.size(); ++j)
+    {
+        const int     b = ty_bytes(p.in_ty[j]);
+        std::uint64_t v = 0;
+        std::memcpy(&v, static_cast<const char*>(in[j]) + t * b, b);
+        R[p.in_reg[j]] = v;
+    }
+    std::uint32_t e = 0, s = 0;
+    run_blocks(p, R.data(), e, ~std::uint64_t(0),
+               [&](std::size_t f)
+               {
+                   first[s] = static_cast<int32_t>(f);
+                   errs[s]  = e;
+                   std::copy(R.begin(), R.end(), regs + std::size_t(s) * nr);
+                   return ++s < max_steps;
+               });
+    *steps = s;
+    return 0;
+}
+
+/// Space-separated names of the labels that start at statement `stmt`
+/// ("START" first for the first statement).
+int
+oracle_block_labels(int k, int32_t stmt, char* buf, size_t len)
+{
+    if(k < 0 || k >= g_count || !buf || !len) return 1;
+    const Program& p = programs()[k];
+    if(stmt < 0 || static_cast<std::size_t>(stmt) >= p.code.size()) return 1;
+    const std::string l = labels_before(p, static_cast<std::size_t>(stmt));
+    const std::size_t n = std::min(len - 1, l.size());
+    std::memcpy(buf, l.data(), n);
+    buf[n] = 0;
+    return 0;
+}
+
+/// Name of register r (oracle register order).
+int
+oracle_register_name(int k, int r, char* buf, size_t len)
+{
+    if(k < 0 || k >= g_count || !buf || !len) return 1;
+    const Program& p = programs()[k];
+    if(r < 0 || static_cast<std::size_t>(r) >= p.reg_name.size()) return 1;
+    const std::string& l = p.reg_name[r];
+    const std::size_t  n = std::min(len - 1, l.size());
+    std::memcpy(buf, l.data(), n);
+    buf[n] = 0;
+    return 0;
+}
+
 int
 oracle_parse_f32(const char* s, float* v)
 {

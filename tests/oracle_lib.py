@@ -37,6 +37,10 @@ def lib():
         L.oracle_smallpt9_direct.argtypes = [C.c_void_p] * 5 + [C.c_uint64, C.c_uint64, C.c_int]
         L.oracle_parse_u64.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
         L.oracle_parse_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
+        L.oracle_trace.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint32,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
+        L.oracle_block_labels.argtypes = [C.c_int, C.c_int32, C.c_char_p, C.c_size_t]
+        L.oracle_register_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -81,3 +85,32 @@ def smallpt9_direct(sid, job, threads=8):
     lib().oracle_smallpt9_direct(sid.ctypes.data, job.ctypes.data, outs[0].ctypes.data,
                                  outs[1].ctypes.data, outs[2].ctypes.data, 0, n, threads)
     return outs
+
+
+def trace(name, inputs, thread, max_steps):
+    """Step trace of one thread (oracle): a list of (label set, error bits,
+    {register name: bits}) after every block."""
+    k, itys, _ = io(name)
+    L = lib()
+    ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
+    ni, no, nr = C.c_int(), C.c_int(), C.c_int()
+    it, ot = (C.c_int * 16)(), (C.c_int * 16)()
+    L.oracle_kernel_io(k, C.byref(ni), C.byref(no), it, ot, C.byref(nr))
+    names = []
+    buf = C.create_string_buffer(256)
+    for r in range(nr.value):
+        assert L.oracle_register_name(k, r, buf, 256) == 0
+        names.append(buf.value.decode())
+    first = np.zeros(max_steps, np.int32)
+    regs = np.zeros((max_steps, nr.value), np.uint64)
+    errs = np.zeros(max_steps, np.uint32)
+    steps = C.c_uint32()
+    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
+    assert L.oracle_trace(k, ip, thread, max_steps, first.ctypes.data, regs.ctypes.data,
+                          errs.ctypes.data, C.byref(steps)) == 0
+    out = []
+    for s in range(steps.value):
+        assert L.oracle_block_labels(k, int(first[s]), buf, 256) == 0
+        out.append((set(buf.value.decode().split()), int(errs[s]),
+                    dict(zip(names, (int(v) for v in regs[s])))))
+    return out
