@@ -103,7 +103,7 @@ run_job(const BatchArgs& a, std::uint32_t j, const CtlArgs& c, WaveCtx& w, std::
 
     for(;;)
     {
-        flush_if_high(w, c.max_steps, c);
+        flush_if_high(w);
         const std::uint64_t live = ballot(has);
         if(~live)
         {
@@ -147,7 +147,7 @@ template <int P, typename... Ks>
 __global__ __launch_bounds__(64) void
 batch_kernel(CtlArgs c, BatchArgs a)
 {
-    WaveCtx w(c.max_steps);
+    WaveCtx w(c);
     std::uint64_t admitted = 0;
     const std::uint32_t nj = a.njobs;
     // each wave walks the job list from its own start, so jobs of every size
@@ -169,7 +169,7 @@ batch_kernel(CtlArgs c, BatchArgs a)
             }(),
             ...);
     }
-    flush_stats(w, c.max_steps, c);
+    flush_stats(w);
     if(c.stats && lane_id() == 0)
         atomicAdd(&stat_slot(c)[ST_ADMITTED], static_cast<unsigned long long>(admitted));
     raise_errors(w.err, w.over, c);

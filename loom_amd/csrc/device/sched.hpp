@@ -159,22 +159,150 @@ uniform64(std::uint64_t v)
     return (std::uint64_t(hi) << 32) | lo;
 }
 
-/// @brief Per-wave scheduling state.  Counters of useful work are kept per
-/// lane (a VALU add under the block's EXEC mask, no popcount) and summed
-/// over the wave when flushed; the dispatch budget counts down.
+/// A flush happens at the first checkpoint with slots >= kFlushSlots.  Until
+/// then a lane's ops are at most slots/64, thread steps at most slots, and
+/// flops at most twice the ops.  Between checkpoints a wave makes at most
+/// kSegment dispatches plus one sweep (at most 64 blocks) of at most 2^16
+/// ops each, so slots grow by less than 192 · 64 · 2^16 < 2^30 and no
+/// 32-bit counter wraps.
+inline constexpr std::uint32_t kFlushSlots = 1u << 30;
+/// Dispatches between checkpoints.  The policies that pick one block at a
+/// time check after every dispatch, so a segment never passes the budget;
+/// the sweep policy checks after every sweep, so the budget trips at most
+/// one sweep late.
+inline constexpr std::uint32_t kSegment = 128;
+
+/// Blocks of at least this many ops count their thread steps with a scalar
+/// popcount of the block mask; smaller blocks (whose waves are short of
+/// scalar issue, not of vector registers) with a per-lane add.
+inline constexpr std::uint32_t kScalarStepOps = 8;
+
+/// @brief Per-wave scheduling state.  Useful work is counted per lane (a
+/// VALU add under the block's EXEC mask) and summed over the wave when
+/// flushed; dispatches and slots are counted in scalar registers, and the
+/// budget is checked at checkpoints (after every kSegment dispatches, or
+/// after every sweep of the sweep policy).
 struct WaveCtx
 {
-    std::uint64_t slots  = 0; ///< wave-uniform: sum of 64*ops(B) over dispatches
-    std::uint64_t budget = 0; ///< wave-uniform: dispatches left before the step budget trips
-    std::uint64_t issued = 0; ///< wave-uniform: dispatches already flushed
-    std::uint32_t lane_ops   = 0; ///< this lane: sum of ops(B) of the blocks it ran
-    std::uint32_t lane_steps = 0; ///< this lane: blocks it ran
-    std::uint32_t lane_flops = 0; ///< this lane: sum of flops(B) of the blocks it ran
-    std::uint32_t err        = 0; ///< this lane: error bits
-    bool          over       = false;
+    std::uint64_t  rem        = 0; ///< wave-uniform: budget left at the last checkpoint
+    std::uint64_t  issued     = 0; ///< wave-uniform: dispatches up to the last checkpoint
+    std::uint64_t  flushed    = 0; ///< wave-uniform: dispatches already in the run's counters
+    std::uint64_t* stats      = nullptr;
+    std::uint32_t  disp       = 0; ///< wave-uniform: dispatches since the last checkpoint
+    std::uint32_t  left       = 0; ///< wave-uniform: dispatches of the current segment
+    std::uint32_t  slots      = 0; ///< wave-uniform: Σ 64·ops(B) since the last flush
+    std::uint32_t  steps      = 0; ///< wave-uniform: Σ popcount(EXEC) of large blocks since the last flush
+    std::uint32_t  lane_steps = 0; ///< this lane: runs of small blocks
+    std::uint32_t  lane_ops   = 0; ///< this lane: Σ ops(B) of the blocks it ran
+    std::uint32_t  lane_flops = 0; ///< this lane: Σ flops(B) of the blocks it ran
+    std::uint32_t  err        = 0; ///< this lane: error bits
+    bool           over       = false;
 
-    LOOM_DEVICE explicit WaveCtx(std::uint64_t max_steps) : budget(max_steps ? max_steps : 1) {}
+    LOOM_DEVICE explicit WaveCtx(const CtlArgs& c) :
+        rem(c.max_steps ? c.max_steps : 1), stats(c.stats),
+        left(static_cast<std::uint32_t>(rem < kSegment ? rem : kSegment))
+    {
+    }
 };
+
+/// @brief This workgroup's copy of the counters.
+LOOM_DEVICE unsigned long long*
+stat_slot(std::uint64_t* stats)
+{
+    return reinterpret_cast<unsigned long long*>(stats + (blockIdx.x % kStatSlots) * kNumStats);
+}
+
+LOOM_DEVICE unsigned long long*
+stat_slot(const CtlArgs& c)
+{
+    return stat_slot(c.stats);
+}
+
+/// @brief Add @p v to counter @p i of a copy (no atomic for zero).
+LOOM_DEVICE void
+stat_add(unsigned long long* st, std::size_t i, std::uint64_t v)
+{
+    if(v) atomicAdd(&st[i], static_cast<unsigned long long>(v));
+}
+
+/// @brief Sum of a per-lane u32 over the wave, whose total must fit 32
+/// bits (every lane must be active).
+LOOM_DEVICE std::uint32_t
+wave_sum(std::uint32_t v)
+{
+    for(int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return uniform(v);
+}
+
+/// @brief Add the wave's counters to the run's and clear them.  Called with
+/// every lane active: at the end of a kernel and at checkpoints.
+LOOM_DEVICE void
+flush_stats(WaveCtx& w)
+{
+    if(w.stats)
+    {
+        // a lane's ops are below slots/64 < 2^25 and its flops below twice
+        // that, so the wave totals fit 32 bits
+        const std::uint64_t act   = wave_sum(w.lane_ops);
+        const std::uint64_t flops = wave_sum(w.lane_flops);
+        const std::uint64_t steps = std::uint64_t(w.steps) + wave_sum(w.lane_steps);
+        const std::uint64_t total = w.issued + w.disp;
+        if(lane_id() == 0)
+        {
+            auto* st = stat_slot(w.stats);
+            stat_add(st, ST_ACTIVE_OPS, act);
+            stat_add(st, ST_SLOT_OPS, w.slots);
+            stat_add(st, ST_DISPATCHES, total - w.flushed);
+            stat_add(st, ST_THREAD_STEPS, steps);
+            stat_add(st, ST_ACTIVE_FLOPS, flops);
+        }
+        w.flushed = total;
+    }
+    w.slots      = 0;
+    w.steps      = 0;
+    w.lane_steps = 0;
+    w.lane_ops   = 0;
+    w.lane_flops = 0;
+}
+
+/// @brief Flush when the slot count says a lane field may fill up.
+LOOM_DEVICE void
+flush_if_high(WaveCtx& w)
+{
+    if(__builtin_expect(w.slots >= kFlushSlots, 0)) flush_stats(w);
+}
+
+/// @brief Fold the dispatches since the last checkpoint into the budget (the
+/// step budget trips once the wave has made max_steps dispatches), flush
+/// before a counter field can overflow, and start the next segment.  Every
+/// lane must be active.
+LOOM_DEVICE void
+checkpoint(WaveCtx& w)
+{
+    flush_if_high(w);
+    // rem <= disp, and rem -= disp, in 32-bit halves (scalar instructions)
+    const std::uint32_t hi = static_cast<std::uint32_t>(w.rem >> 32);
+    const std::uint32_t lo = static_cast<std::uint32_t>(w.rem);
+    w.issued += w.disp;
+    if(hi == 0 && lo <= w.disp)
+    {
+        w.disp = 0;
+        w.rem  = 0;
+        w.over = true;
+        return;
+    }
+    w.rem -= w.disp;
+    w.disp = 0;
+    const bool small = (w.rem >> 32) == 0 && static_cast<std::uint32_t>(w.rem) < kSegment;
+    w.left           = small ? static_cast<std::uint32_t>(w.rem) : kSegment;
+}
+
+/// @brief Checkpoint at the end of a segment (after a dispatch or a sweep).
+LOOM_DEVICE void
+segment_check(WaveCtx& w)
+{
+    if(__builtin_expect(w.disp >= w.left, 0)) checkpoint(w);
+}
 
 /// @brief Step result: the next block (or rank) in the low bits, and
 /// kMayHalt when the block just run has HALT among its successors (only
@@ -188,16 +316,17 @@ LOOM_DEVICE void
 run_block(Regs<K>& r, std::uint32_t& pc, WaveCtx& w)
 {
     constexpr ir::Block blk = ir::table_of<K>.block[B];
+    static_assert(blk.flops <= 2u * blk.ops, "flops exceed twice the op count");
+    if constexpr(blk.ops >= kScalarStepOps) w.steps += popc(ballot(pc == B));
     if(pc == B)
     {
         exec_block<K, B>(r, pc, w.err);
+        if constexpr(blk.ops < kScalarStepOps) w.lane_steps += 1;
         w.lane_ops += blk.ops;
-        w.lane_steps += 1;
         if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
     }
-    w.slots += 64ull * blk.ops;
-    w.budget -= 1;
-    if(w.budget == 0) w.over = true;
+    w.slots += 64u * blk.ops;
+    w.disp += 1;
 }
 
 /// @brief True when HALT is a successor of block @p B.
@@ -362,6 +491,7 @@ run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t
         {
             const std::uint32_t ran = cur;
             cur = dispatch<K>(cur, r, pc, pending, thr, w);
+            segment_check(w);
             if(w.over) return;
             if(live_thr && ((LOOM_CX(halt_blocks<K>) >> ran) & 1u) &&
                popc(ballot(pc != HALT)) <= live_thr)
@@ -542,6 +672,7 @@ run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, Wave
     for(;;)
     {
         const std::uint32_t res = dispatch_topo<K>(cur, r, pc, present, w);
+        segment_check(w);
         if(w.over || res == NONE) return;
         if(live_thr && (res & kMayHalt) && popc(ballot(pc != HALT)) <= live_thr) return;
         cur = res & kIndex;
@@ -569,6 +700,7 @@ run_sweep(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, Wav
                 }(),
                 ...);
         }(std::make_index_sequence<ir::table_of<K>.nb>{});
+        segment_check(w);
         if(w.over) return;
         const std::uint64_t live = ballot(pc != HALT);
         if(!live || (live_thr && popc(live) <= live_thr)) return;
@@ -588,63 +720,6 @@ run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_
         run_sweep<K>(r, pc, live_thr, force, w);
     else
         run_to_threshold<K>(r, pc, thr, live_thr, force, w);
-}
-
-/// @brief This workgroup's copy of the counters.
-LOOM_DEVICE unsigned long long*
-stat_slot(const CtlArgs& c)
-{
-    return reinterpret_cast<unsigned long long*>(c.stats + (blockIdx.x % kStatSlots) * kNumStats);
-}
-
-/// @brief Add @p v to counter @p i of a copy (no atomic for zero).
-LOOM_DEVICE void
-stat_add(unsigned long long* st, std::size_t i, std::uint64_t v)
-{
-    if(v) atomicAdd(&st[i], static_cast<unsigned long long>(v));
-}
-
-/// @brief Sum of a per-lane u32 over the wave (every lane must be active).
-LOOM_DEVICE std::uint64_t
-wave_sum(std::uint32_t v)
-{
-    std::uint64_t s = v;
-    for(int off = 32; off >= 1; off >>= 1)
-        s += static_cast<std::uint64_t>(__shfl_xor(static_cast<unsigned long long>(s), off, 64));
-    return uniform64(s);
-}
-
-/// @brief Add the wave's counters to the run's and clear them.  Called with
-/// every lane active: at the end of a kernel, and when a lane's counters
-/// approach their 32-bit range in long persistent runs.
-LOOM_DEVICE void
-flush_stats(WaveCtx& w, std::uint64_t max_steps, const CtlArgs& c)
-{
-    if(!c.stats) return;
-    const std::uint64_t act = wave_sum(w.lane_ops), steps = wave_sum(w.lane_steps),
-                        flops = wave_sum(w.lane_flops);
-    const std::uint64_t disp = (max_steps ? max_steps : 1) - w.budget - w.issued;
-    if(lane_id() == 0)
-    {
-        auto* st = stat_slot(c);
-        stat_add(st, ST_ACTIVE_OPS, act);
-        stat_add(st, ST_SLOT_OPS, w.slots);
-        stat_add(st, ST_DISPATCHES, disp);
-        stat_add(st, ST_THREAD_STEPS, steps);
-        stat_add(st, ST_ACTIVE_FLOPS, flops);
-    }
-    w.issued += disp;
-    w.slots      = 0;
-    w.lane_ops   = 0;
-    w.lane_steps = 0;
-    w.lane_flops = 0;
-}
-
-/// @brief Flush when some lane's counters passed half their range.
-LOOM_DEVICE void
-flush_if_high(WaveCtx& w, std::uint64_t max_steps, const CtlArgs& c)
-{
-    if(ballot(w.lane_ops >= 0x80000000u)) flush_stats(w, max_steps, c);
 }
 
 LOOM_DEVICE void
@@ -677,10 +752,10 @@ compact_kernel(IoArgs io, CtlArgs c)
         load_inputs<K>(r, io.in, tid);
         pc = 0;
     }
-    WaveCtx w(c.max_steps);
+    WaveCtx w(c);
     run_policy<K, P>(r, pc, 0, 0, true, w);
     if(valid) store_outputs<K>(r, io.out, tid);
-    flush_stats(w, c.max_steps, c);
+    flush_stats(w);
     raise_errors(w.err, w.over, c);
 }
 
@@ -973,7 +1048,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     char*          qout = h.q[par ^ 1u];
     std::uint32_t* qcnt = ctl->qcount[par ^ 1u];
 
-    WaveCtx w(c.max_steps);
+    WaveCtx w(c);
     std::uint64_t spill_rows = 0, spill_bytes = 0, fill_rows = 0, fill_bytes = 0,
                   admitted = 0, refills = 0, batches = 0, spill_failed = 0;
     Reservoir res;
@@ -985,7 +1060,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 
     for(;;)
     {
-        flush_if_high(w, c.max_steps, c);
+        flush_if_high(w);
         // ---- claim a batch (the wave is empty here) ----
         pc              = HALT;
         has             = false;
@@ -1102,7 +1177,7 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         }
         if(w.over) break;
     }
-    flush_stats(w, c.max_steps, c);
+    flush_stats(w);
     if(c.stats && lane == 0)
     {
         auto* st = stat_slot(c);
