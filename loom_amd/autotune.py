@@ -29,10 +29,11 @@ DEFAULT = {"live_threshold": 48, "regroup": False, "reconv": 2, "waves_per_cu": 
            "threshold": 0, "refill": True}
 
 SPACE = {
-    "live_threshold": [32, 40, 48, 56],
+    # 0: refill only once every lane of the wave halted
+    "live_threshold": [0, 32, 40, 48, 56],
     "regroup": [False, True],
     "reconv": [0, 1, 2],
-    "waves_per_cu": [0, 16, 20],
+    "waves_per_cu": [0, 8, 16, 20],
 }
 
 

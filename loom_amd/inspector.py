@@ -5,9 +5,9 @@ recorded on the GPU (--trace), printed as the reference's step-by-step
 inspector prints it: one row per block, the block label and the tracked
 registers (src/vm/dbg.hpp:1099-1140, rows as print_state 920-935).
 
-    python -m loom_amd.inspect smallpt9_f32            # listing (no GPU needed)
-    python -m loom_amd.inspect collatz_u64 --run 65536 # + resources and counters
-    python -m loom_amd.inspect fib_u64 --trace 10 --steps 40 --track n,c
+    python -m loom_amd.inspector smallpt9_f32            # listing (no GPU needed)
+    python -m loom_amd.inspector collatz_u64 --run 65536 # + resources and counters
+    python -m loom_amd.inspector fib_u64 --trace 10 --steps 40 --track n,c
 """
 from __future__ import annotations
 

@@ -75,7 +75,7 @@ def test_calls_before_init_fail_cleanly():
 def test_kernel_listing():
     """The lowered listing of fib names its three blocks, their successors
     and the loop's instructions (no device needed)."""
-    from loom_amd import inspect
+    from loom_amd import inspector as inspect
     text = inspect.listing("fib_u64")
     assert "3 blocks" in text
     assert "block 0 START" in text and "block 1 work" in text and "block 2 loop" in text

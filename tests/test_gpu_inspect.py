@@ -4,7 +4,7 @@ regression guard for code shapes that demote the register file)."""
 import pytest
 
 import loom_amd
-from loom_amd import inspect
+from loom_amd import inspector as inspect
 
 pytestmark = pytest.mark.gpu
 

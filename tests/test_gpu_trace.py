@@ -10,7 +10,7 @@ import pytest
 
 import loom_amd
 from loom_amd import abi
-from loom_amd import inspect as I
+from loom_amd import inspector as I
 from tests import oracle_lib as O
 from tests.test_gpu_parity import CASES
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import loom_amd
-from loom_amd import inspect as I
+from loom_amd import inspector as I
 from tests import oracle_lib as O
 
 FIB = [0, 1]
