@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job mixed batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-regroup", action="store_true", help="skip the queue-regrouping form")
+    ap.add_argument("--no-int", action="store_true", help="skip the fib and Collatz configs")
     ap.add_argument("--qps", type=float, default=SERVE_QPS, help="serving arrival rate")
     ap.add_argument("--requests", type=int, default=SERVE_REQUESTS)
     return ap.parse_args()
@@ -166,10 +167,67 @@ def cpu_baseline():
     t = time.perf_counter()
     oracle_lib.smallpt9_direct(sid, job, threads=threads)
     dt = time.perf_counter() - t
+    # one core on the first 1/16 of that sample, comparable to single-core figures
+    n1 = n // 16
+    t = time.perf_counter()
+    oracle_lib.smallpt9_direct(sid[:n1], job[:n1], threads=1)
+    dt1 = time.perf_counter() - t
     return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"first {n} samples (rows 0-{H // 2 - 1}) of the {W}x{H}x{SPP} frame, "
                       f"direct C++ restatement of the kernel, {threads} threads, "
-                      f"{dt:.2f} s wall"}
+                      f"{dt:.2f} s wall",
+            "single_core": {"value": n1 / dt1, "unit": "samples/s",
+                            "sample": f"first {n1} samples, 1 thread, {dt1:.2f} s wall"},
+            "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def integer_configs(steps: int):
+    """fib u64 over 2^20 threads and Collatz u64 over 2^24 threads, inputs in
+    HBM: the tuned hybrid form and the compact form under the reference
+    policy and the sweep policy, threads/s and active lanes, outputs
+    compared across forms."""
+    import loom_amd
+    from loom_amd import abi, autotune, workloads
+    out = {}
+    for key, name, cols in (("fib_u64_2^20", "fib_u64", [workloads.fib_inputs(1 << 20)]),
+                            ("collatz_u64_2^24", "collatz_u64",
+                             [workloads.collatz_inputs(1 << 24)])):
+        k = loom_amd.kernel(name)
+        n = len(cols[0])
+        din = [loom_amd.DeviceArray.from_numpy(c) for c in cols]
+        forms = {"hybrid": (abi.MODE_HYBRID, loom_amd.make_params(
+                     mode=abi.MODE_HYBRID, **autotune.best_params(name))),
+                 "compact_successor_major": (abi.MODE_COMPACT, loom_amd.make_params(
+                     mode=abi.MODE_COMPACT)),
+                 "compact_sweep": (abi.MODE_COMPACT, loom_amd.make_params(
+                     mode=abi.MODE_COMPACT, reconv=abi.RECONV_SWEEP))}
+        entry, first = {"threads": n}, None
+        for form, (mode, params) in forms.items():
+            dout = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
+            loom_amd.run(k, mode, din, dout, params=params)
+            ms, r = [], None
+            for _ in range(max(steps, 1)):
+                r = loom_amd.run(k, mode, din, dout, params=params)
+                ms.append(r.ms_total)
+            got = [d.numpy() for d in dout]
+            if first is None:
+                first = got
+            entry[form] = {"ms": float(np.median(ms)), "threads_per_s": n / (np.median(ms) / 1e3),
+                           "active_lane_fraction": r.active_lane_fraction,
+                           "outputs_identical": all(np.array_equal(x, y)
+                                                    for x, y in zip(got, first))}
+        out[key] = entry
+    return out
 
 
 def main():
@@ -251,6 +309,8 @@ def main():
 
     if not a.no_regroup:
         result["regroup"] = regroup_leg(k, ins, outs, h_img)
+    if not a.no_int:
+        result["integer_configs"] = integer_configs(a.steps)
     if not a.no_batch:
         result["batch"] = mixed_batch(max(1, min(a.steps, 3)))
     if not a.no_serve:

@@ -194,8 +194,11 @@ int loom_fill(void* dst, const void* pattern, size_t elem_bytes, uint64_t count)
  * issued.  Each call returns a ticket: loom_copy_status reports whether the
  * copy (and every earlier one) completed, loom_copy_wait blocks until it
  * has.  The caller keeps both buffers alive until then; host buffers should
- * be pinned (loom_host_alloc) for the copy to run asynchronously.  Replaces
- * the CUDA test backend's staging copies (src/vm/cuda.hpp:1468-1578). */
+ * be pinned (loom_host_alloc) for the copy to run asynchronously.  Copies are
+ * not ordered after main-stream work still running: runs return once their
+ * kernels finished, so copy a run's outputs after it returned, and start a
+ * run on copied inputs after their ticket completed.  Replaces the CUDA test
+ * backend's staging copies (src/vm/cuda.hpp:1468-1578). */
 enum loom_copy_kind
 {
     LOOM_COPY_H2D = 0,

@@ -202,6 +202,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
     std::deque<std::uint64_t> pending;
     std::vector<Resident>     loading, running, draining;
     std::uint64_t             next = 0, finished = 0;
+    std::uint64_t             last_ticket = 0; // latest copy issued by this run
 
     auto request_bytes = [&](const loom_request& q, std::vector<std::uint64_t>& cols)
     {
@@ -226,9 +227,12 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         const loom_kernel_info& ki = info.at(q.kernel);
         r.ticket                   = 0;
         for(int k = 0; k < ki.n_outputs && q.n; ++k)
+        {
             if(int e = loom_copy_async(q.outputs[k], r.out[k], q.n * type_bytes(ki.output_type[k]),
                                        LOOM_COPY_D2H, &r.ticket))
                 return e;
+            last_ticket = r.ticket;
+        }
         draining.push_back(std::move(r));
         return 0;
     };
@@ -239,6 +243,16 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         return r.ticket ? loom_copy_status(r.ticket, &done) : 0;
     };
 
+    // on an error, let the copies already issued land before the arena is
+    // handed to another run, then give the whole arena back
+    auto fail = [&](int e) -> int
+    {
+        if(last_ticket) loom_copy_wait(last_ticket);
+        loom_synchronize();
+        s->arena.reset(s->cfg.hbm_budget);
+        return e;
+    };
+
     std::vector<std::uint64_t> cols;
     std::vector<loom_job>      jobs;
     while(finished < count)
@@ -247,7 +261,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         for(std::size_t i = 0; i < draining.size();)
         {
             int done = 0;
-            if(int e = ready(draining[i], done)) return e;
+            if(int e = ready(draining[i], done)) return fail(e);
             if(!done)
             {
                 ++i;
@@ -302,14 +316,15 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
                                            loom_fill(r.in[k], &q.scalars[k], tb, q.n) :
                                            loom_copy_async(r.in[k], q.inputs[k], q.n * tb,
                                                            LOOM_COPY_H2D, &r.ticket);
-                if(e) return e;
+                if(e) return fail(e);
+                if(r.ticket) last_ticket = r.ticket;
             }
             q.admit_ms = now();
             pending.pop_front();
             st.peak_bytes = std::max(st.peak_bytes, s->arena.used());
             if(q.n == 0)
             {
-                if(int e = drain(r)) return e;
+                if(int e = drain(r)) return fail(e);
                 continue;
             }
             loading.push_back(std::move(r));
@@ -318,7 +333,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         for(std::size_t i = 0; i < loading.size();)
         {
             int done = 0;
-            if(int e = ready(loading[i], done)) return e;
+            if(int e = ready(loading[i], done)) return fail(e);
             if(!done)
             {
                 ++i;
@@ -334,11 +349,11 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             // nothing to run: wait for a copy, or idle until the next arrival
             if(!loading.empty())
             {
-                if(int e = loom_copy_wait(loading.front().ticket)) return e;
+                if(int e = loom_copy_wait(loading.front().ticket)) return fail(e);
             }
             else if(!draining.empty())
             {
-                if(int e = loom_copy_wait(draining.front().ticket)) return e;
+                if(int e = loom_copy_wait(draining.front().ticket)) return fail(e);
             }
             else if(next < count && realtime)
             {
@@ -367,7 +382,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
                                   &s->cfg.params, &rs))
         {
             for(auto& r: running) reqs[r.req].status = e;
-            return e;
+            return fail(e);
         }
         st.gpu_ms += rs.ms_main;
         ++st.steps;
@@ -379,7 +394,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             ++reqs[running[i].req].steps;
             if(running[i].cursor >= reqs[running[i].req].n)
             {
-                if(int e = drain(running[i])) return e;
+                if(int e = drain(running[i])) return fail(e);
             }
             else
                 keep.push_back(std::move(running[i]));

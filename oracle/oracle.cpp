@@ -137,6 +137,87 @@ oracle_run(int k, const void* const* in, void* const* out, uint64_t begin, uint6
     return 0;
 }
 
+/// Work of threads [begin, begin+count), counted the way the reference
+/// prices a block (simd.hpp:2341-2665 use |B| = #Compute + 1): per executed
+/// block one thread step, #Compute + 1 ops, and one flop per floating-point
+/// Compute other than moves, selects, table reads and checks (two for
+/// fma/fms).  `threads` CPU workers.
+int
+oracle_work(int k, const void* const* in, uint64_t begin, uint64_t count, int threads,
+            uint64_t* steps, uint64_t* ops, uint64_t* flops)
+{
+    if(k < 0 || k >= g_count || !in || !steps || !ops || !flops) return 1;
+    const Program& p = programs()[k];
+    // cost of the block starting at each statement: computes up to the next label
+    std::vector<std::uint32_t> bops(p.code.size(), 0), bflops(p.code.size(), 0);
+    for(std::size_t i = 0; i < p.code.size(); ++i)
+    {
+        if(p.code[i].kind == Stmt::label) continue;
+        std::uint32_t nc = 0, nf = 0;
+        for(std::size_t j = i; j < p.code.size() && p.code[j].kind != Stmt::label; ++j)
+        {
+            const Stmt& s = p.code[j];
+            if(s.kind != Stmt::compute) continue;
+            ++nc;
+            const bool fp = s.ety == Ty::f32 || s.ety == Ty::f64;
+            if(fp && s.code != Code::mov && s.code != Code::cmov && s.code != Code::sct &&
+               s.code != Code::slt && s.code != Code::assert_fp)
+                nf += (s.code == Code::fma || s.code == Code::fms) ? 2 : 1;
+        }
+        bops[i]   = nc + 1;
+        bflops[i] = nf;
+    }
+    std::atomic<std::uint64_t> st{0}, op{0}, fl{0};
+    auto range = [&](std::uint64_t b, std::uint64_t e)
+    {
+        std::vector<std::uint64_t> R(p.reg_name.size());
+        std::uint64_t              ls = 0, lo = 0, lf = 0;
+        std::uint32_t              err = 0;
+        for(std::uint64_t t = b; t < e; ++t)
+        {
+            std::fill(R.begin(), R.end(), 0);
+            for(std::size_t j = 0; j < p.in_reg.size(); ++j)
+            {
+                const int     w = ty_bytes(p.in_ty[j]);
+                std::uint64_t v = 0;
+                std::memcpy(&v, static_cast<const char*>(in[j]) + t * w, w);
+                R[p.in_reg[j]] = v;
+            }
+            run_blocks(p, R.data(), err, ~std::uint64_t(0),
+                       [&](std::size_t f)
+                       {
+                           ++ls;
+                           lo += bops[f];
+                           lf += bflops[f];
+                           return true;
+                       });
+        }
+        st += ls;
+        op += lo;
+        fl += lf;
+    };
+    threads = std::max(1, threads);
+    if(threads == 1 || count < 1024)
+        range(begin, begin + count);
+    else
+    {
+        std::vector<std::thread> pool;
+        const std::uint64_t      chunk = (count + threads - 1) / threads;
+        for(int w = 0; w < threads; ++w)
+        {
+            const std::uint64_t b = begin + chunk * w;
+            const std::uint64_t e = std::min(begin + count, b + chunk);
+            if(b >= e) break;
+            pool.emplace_back(range, b, e);
+        }
+        for(auto& th: pool) th.join();
+    }
+    *steps = st;
+    *ops   = op;
+    *flops = fl;
+    return 0;
+}
+
 /// Step trace of thread t of host columns: after every block, first[s] = the
 /// index of the block's first statement, errs[s] = error bits so far and
 /// regs[s * nregs + r] = bits of register r (oracle register order).

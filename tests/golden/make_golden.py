@@ -1,6 +1,7 @@
 """Writes the golden fixtures of this directory.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py work     # work_totals.json only
 
 The reference holds no golden vectors for this path and cannot be built
 here, so the fixtures come from computations independent of the oracle's
@@ -17,6 +18,12 @@ block interpreter and of the GPU backend:
 * literals.json — literal strings with the correctly rounded float32 and
   float64 values (Python's parser) beside the values of the reference's
   literal parser as restated by the oracle.
+* work_totals.json — the algorithmic work of whole configurations (thread
+  block steps, ops = #Compute + 1 per block run, floating-point ops with
+  fma = 2): fib and Collatz in closed form from the IR text and the
+  brute-force step counts (Collatz: odd steps counted separately), the path
+  tracer (prefixes of the 1024x768x64 frame and the whole frame) by the
+  oracle's block interpreter.
 
 Each file stores SHA-256 digests of the full input and output columns and a
 sample of (index, input, output) rows.
@@ -82,6 +89,57 @@ def collatz_steps(x: np.ndarray) -> np.ndarray:
     return steps
 
 
+def collatz_odd_steps(x: np.ndarray) -> np.ndarray:
+    """Number of 3x+1 steps of each start value (vectorised)."""
+    x = x.astype(np.uint64).copy()
+    odd_n = np.zeros(len(x), np.uint64)
+    idx = np.flatnonzero(x > 1)
+    cur = x[idx]
+    while len(idx):
+        odd = (cur & np.uint64(1)).astype(bool)
+        odd_n[idx[odd]] += 1
+        cur = np.where(odd, cur * np.uint64(3) + np.uint64(1), cur >> np.uint64(1))
+        keep = cur > 1
+        idx, cur = idx[keep], cur[keep]
+    return odd_n
+
+
+def work_fixture(full_frame: bool = True):
+    """Work totals; see the module docstring."""
+    from tests import oracle_lib as O
+    out = {}
+    # fib (programs.hpp): START 3 ops; n > 1: work 4 ops, then n-1 loop trips of 6
+    n = workloads.fib_inputs(1 << 20).astype(np.int64)
+    big = n > 1
+    out["fib_u64_c1"] = {
+        "threads": int(len(n)),
+        "steps": int(np.where(big, n + 1, 1).sum()),
+        "ops": int(np.where(big, 7 + 6 * (n - 1), 3).sum()),
+        "flops": 0,
+        "how": "closed form: steps 1 or n+1, ops 3 or 7+6(n-1)"}
+    # Collatz: START 3 + done 2, per step: step 2 + next 3 + (even 2 | odd3 3)
+    x = workloads.collatz_inputs(1 << 24)
+    cnt = collatz_steps(x).astype(np.uint64)
+    odd = collatz_odd_steps(x)
+    out["collatz_u64_c2"] = {
+        "threads": int(len(x)),
+        "steps": int((2 + 3 * cnt).sum(dtype=np.uint64)),
+        "ops": int((5 + 7 * cnt + odd).sum(dtype=np.uint64)),
+        "flops": 0,
+        "how": "closed form from brute-force step counts: steps 2+3c, ops 5+7c+odd"}
+    sid, job = workloads.pt_inputs(1024, 768, 64)
+    sizes = [1 << 16, 1 << 20] + ([len(sid)] if full_frame else [])
+    for m in sizes:
+        st, op, fl = O.work("smallpt9_f32", [sid[:m], job[:m]])
+        key = "smallpt9_c3" if m == len(sid) else f"smallpt9_c3_first_{m}"
+        out[key] = {"threads": m, "steps": st, "ops": op, "flops": fl,
+                    "how": "oracle block interpreter, " + (
+                        "the whole 1024x768x64 frame" if m == len(sid) else
+                        "first samples of the 1024x768x64 frame")}
+        print(key, out[key], flush=True)
+    return out
+
+
 def collatz_fixture():
     x = workloads.collatz_inputs(1 << 24)
     s = collatz_steps(x)
@@ -127,6 +185,10 @@ def literal_fixture():
 
 
 def main():
+    if sys.argv[1:] == ["work"]:
+        (HERE / "work_totals.json").write_text(json.dumps(work_fixture(), indent=1) + "\n")
+        print("work totals", flush=True)
+        return
     (HERE / "fib_u64_c1.json").write_text(json.dumps(fib_fixture(), indent=0) + "\n")
     print("fib", flush=True)
     (HERE / "collatz_u64_c2.json").write_text(json.dumps(collatz_fixture(), indent=0) + "\n")
@@ -135,6 +197,8 @@ def main():
     print("smallpt9", flush=True)
     (HERE / "literals.json").write_text(json.dumps(literal_fixture(), indent=0) + "\n")
     print("literals", flush=True)
+    (HERE / "work_totals.json").write_text(json.dumps(work_fixture(), indent=1) + "\n")
+    print("work totals", flush=True)
 
 
 if __name__ == "__main__":

@@ -41,6 +41,9 @@ def lib():
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
         L.oracle_block_labels.argtypes = [C.c_int, C.c_int32, C.c_char_p, C.c_size_t]
         L.oracle_register_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+        L.oracle_work.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint64, C.c_int,
+                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -114,3 +117,14 @@ def trace(name, inputs, thread, max_steps):
         out.append((set(buf.value.decode().split()), int(errs[s]),
                     dict(zip(names, (int(v) for v in regs[s])))))
     return out
+
+
+def work(name, inputs, threads=8):
+    """(thread steps, ops, flops) of every thread of `inputs` (oracle)."""
+    k, itys, _ = io(name)
+    ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
+    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
+    st, op, fl = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    assert lib().oracle_work(k, ip, 0, len(ins[0]), threads, C.byref(st), C.byref(op),
+                             C.byref(fl)) == 0
+    return st.value, op.value, fl.value

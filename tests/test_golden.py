@@ -92,3 +92,32 @@ def test_ppm_of_fixture_frame(tmp_path):
     px = np.frombuffer(raw[len(head):], np.uint8).reshape(24, 32, 3)
     assert px[:3].mean() > px[-3:].mean()  # light near the top row of the file
     assert np.array_equal(px, image.to_bytes(img[::-1]))
+
+
+def test_work_totals_fib_full_config():
+    """The oracle's count of blocks, ops and flops over the whole fib config
+    equals the closed form of the fixture."""
+    f = load("work_totals.json")["fib_u64_c1"]
+    assert f["threads"] == 1 << 20
+    got = O.work("fib_u64", [workloads.fib_inputs(1 << 20)])
+    assert got == (f["steps"], f["ops"], f["flops"])
+
+
+def test_work_totals_collatz_closed_form_on_prefix():
+    """On a prefix of the Collatz config the oracle's count equals the closed
+    form (steps 2 + 3c, ops 5 + 7c + odd steps) of brute-force step counts."""
+    from tests.golden.make_golden import collatz_odd_steps, collatz_steps
+    x = workloads.collatz_inputs(1 << 24)[: 1 << 14]
+    c = collatz_steps(x).astype(np.uint64)
+    odd = collatz_odd_steps(x)
+    got = O.work("collatz_u64", [x])
+    assert got == (int((2 + 3 * c).sum()), int((5 + 7 * c + odd).sum()), 0)
+
+
+def test_work_totals_path_tracer_prefix():
+    f = load("work_totals.json")["smallpt9_c3_first_65536"]
+    sid, job = workloads.pt_inputs(1024, 768, 64)
+    got = O.work("smallpt9_f32", [sid[:65536], job[:65536]])
+    assert got == (f["steps"], f["ops"], f["flops"])
+    # every sample runs START, then at least one intersection
+    assert f["steps"] >= 2 * 65536 and f["flops"] > 0

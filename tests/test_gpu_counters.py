@@ -55,3 +55,49 @@ def test_schedule_free_counters_agree(gpu, name, n):
         if ref is None:
             ref = got
         assert got == ref, (name, fname)
+
+
+def _work(key):
+    import json
+    from pathlib import Path
+    p = Path(__file__).resolve().parent / "golden" / "work_totals.json"
+    return json.loads(p.read_text()).get(key)
+
+
+WORK_FORMS = [
+    ("hybrid_tuned", abi.MODE_HYBRID, None),
+    ("compact_sm", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+]
+
+
+@pytest.mark.parametrize("key,name", [("fib_u64_c1", "fib_u64"), ("collatz_u64_c2", "collatz_u64"),
+                                      ("smallpt9_c3_first_1048576", "smallpt9_f32"),
+                                      ("smallpt9_c3", "smallpt9_f32")])
+def test_counters_equal_work_totals(gpu, key, name):
+    """The device counters of whole configurations (the numerators of the
+    benchmark's active-lane fraction and roofline) equal the committed
+    totals: closed forms for fib and Collatz, the oracle's count for the
+    path tracer."""
+    from loom_amd import autotune
+    f = _work(key)
+    if f is None:
+        pytest.skip(f"no {key} totals in the fixture")
+    n = f["threads"]
+    if name == "fib_u64":
+        ins = [workloads.fib_inputs(n)]
+    elif name == "collatz_u64":
+        ins = [workloads.collatz_inputs(n)]
+    else:
+        ins = [a[:n] for a in workloads.pt_inputs(1024, 768, 64)]
+    dev = [loom_amd.DeviceArray.from_numpy(a) for a in ins]
+    k = loom_amd.kernel(name)
+    for fname, mode, params in WORK_FORMS:
+        p = autotune.best_params(name) if params is None else params
+        print(f"[gpu] work totals {key} {fname} n={n}", flush=True)
+        outs = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
+        r = loom_amd.run(k, mode, dev, outs, params=loom_amd.make_params(mode=mode, **p))
+        assert r.errors == 0
+        c = r.counters
+        assert (c[abi.C_THREAD_STEPS], c[abi.C_ACTIVE_OPS], c[abi.C_ACTIVE_FLOPS]) == \
+            (f["steps"], f["ops"], f["flops"]), (key, fname)
