@@ -154,6 +154,19 @@ def traffic_from_profile():
     return None
 
 
+def valu_ceiling_from_profile():
+    """Measured non-packed FP32 fma rate (TFLOP/s) of the committed VALU probe
+    summary; None if it is absent."""
+    p = ROOT / "profiles" / "valu_rate_probe.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        f = line.split()
+        if line.strip().startswith("fp32 fma") and "TFLOP/s" in f:
+            return float(f[f.index("TFLOP/s") - 1])
+    return None
+
+
 def cpu_baseline():
     """The oracle's direct C++ path tracer (compiled scalar restatement),
     one thread per host core (at most 16), on the first half of the frame
@@ -304,8 +317,13 @@ def main():
         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profile(),
         "kernel": f"hybrid_pass_kernel<smallpt9_f32, {POLICY_NAMES[hyb['reconv']]}>",
         "launch_ms": launch_ms, "launches_per_step": launches,
+        "measured_vector_fp32_ceiling": valu_ceiling_from_profile(),
         "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
-                "(fma = 2) per launch over the launch time, against the FP32 vector peak"}
+                "(fma = 2) per launch over the launch time, against the FP32 vector peak; "
+                "measured_vector_fp32_ceiling is the non-packed fma rate of the VALU probe "
+                "(TFLOP/s)"}
+    ceil = result["roofline"]["measured_vector_fp32_ceiling"]
+    result["roofline"]["frac_of_measured_ceiling"] = achieved / ceil if ceil else None
 
     if not a.no_regroup:
         result["regroup"] = regroup_leg(k, ins, outs, h_img)
