@@ -107,7 +107,9 @@ typedef struct loom_params
     uint32_t waves_per_cu;   /* hybrid persistent grid; 0 = occupancy limit */
     uint32_t collect_stats;  /* fill loom_run_stats.counters */
     uint64_t queue_capacity; /* hybrid rows per label queue; 0 = automatic */
-    uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40 */
+    uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40.
+                                Checked after every dispatch (reconv 0, 1)
+                                or after every sweep (reconv 2) */
 } loom_params;
 /* A NULL loom_params* means: threshold 0, refill on, regroup off, no stats,
  * successor-major for compact, sweep with live_threshold 48 for hybrid and
