@@ -82,7 +82,7 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
 
 
 BATCH_SPACE = {
-    "live_threshold": [32, 48, 56],
+    "live_threshold": [0, 32, 48, 56],
     "waves_per_cu": [0, 12, 16],
     "reconv": [1, 2],
 }
