@@ -12,6 +12,8 @@
 /// `vm/lower.hpp`, which keeps compile time linear in the kernel size.
 #pragma once
 
+#include "lists.hpp"
+
 #include <array>
 #include <cstddef>
 #include <cstdint>
@@ -84,25 +86,9 @@ struct type_sequence
     static constexpr std::size_t size = sizeof...(Ts);
 };
 
-namespace intlist
-{
-namespace detail
-{
+/// @brief A two-type list (reference: `loom::type_pair`).
 template <typename A, typename B>
-struct join_impl;
-
-template <char... A, char... B>
-struct join_impl<char_sequence<A...>, char_sequence<B...>>
-{
-    using type = char_sequence<A..., B...>;
-};
-} // namespace detail
-
-/// @brief Concatenate two char sequences (used by `CS_JOIN` for prefixed
-/// labels and register arrays).
-template <typename A, typename B>
-using join = typename detail::join_impl<A, B>::type;
-} // namespace intlist
+using type_pair = type_sequence<A, B>;
 
 namespace detail
 {

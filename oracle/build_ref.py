@@ -1,0 +1,85 @@
+"""ORACLE / TEST INFRASTRUCTURE: build the reference's own path-tracing
+kernels (oracle/ref_kernels.hpp) against this repository's front-end, from
+the reference checkout where it lies, into oracle/_ref/:
+
+* liboracle_ref.so — the oracle interpreter (oracle.cpp) over those kernels;
+* libloom_refkernels.so — the same kernels lowered for gfx950 by the
+  product's backend; loading it (after the product library) registers them
+  with the product library, where the C ABI finds them by name.
+
+Only tests load these.  Nothing is built when the reference is absent (the
+GPU box has none; the libraries built here travel with the tree).
+
+    python oracle/build_ref.py [--ref /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+OUT = HERE / "_ref"
+ORACLE_LIB = OUT / "liboracle_ref.so"
+KERNEL_LIB = OUT / "libloom_refkernels.so"
+SOURCES = ["ref_pt_const.hip", "ref_pt2_const.hip", "ref_pt2_const_1024.hip"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd[:3])} ... failed\n{r.stderr[-6000:]}")
+
+
+def build(ref: Path) -> list[Path]:
+    sys.path.insert(0, str(ROOT))
+    from loom_amd import build as B
+    src = ref / "src"
+    if not (src / "smallpt" / "vm" / "pt.hpp").exists():
+        return []
+    OUT.mkdir(parents=True, exist_ok=True)
+
+    def oracle():
+        tmp = ORACLE_LIB.with_suffix(".tmp")
+        _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",
+              "-fno-fast-math", "-DLOOM_ORACLE_REF", f"-I{ROOT / 'include' / 'loom_vm'}",
+              f"-I{ROOT / 'include'}", f"-I{HERE}", f"-I{src}", "-shared", "-o", str(tmp),
+              str(HERE / "oracle.cpp"), "-lpthread"])
+        os.replace(tmp, ORACLE_LIB)
+        return ORACLE_LIB
+
+    def obj(name):
+        o = OUT / (Path(name).stem + ".o")
+        _run([B.HIPCC, *B.COMMON, f"-I{ROOT / 'loom_amd' / 'csrc'}", f"-I{HERE}", f"-I{src}",
+              "-x", "hip", "-c", str(HERE / name), "-o", str(o)])
+        return o
+
+    with cf.ThreadPoolExecutor(len(SOURCES) + 1) as ex:
+        fo = ex.submit(oracle)
+        objs = list(ex.map(obj, SOURCES))
+        fo.result()
+    tmp = KERNEL_LIB.with_suffix(".tmp")
+    _run([B.HIPCC, f"--offload-arch={B.ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          f"-L{B.BUILD}", "-lloom_amd", f"-Wl,-rpath,{B.BUILD}",
+          "-Wl,-rpath,$ORIGIN/../../loom_amd/_build"])
+    os.replace(tmp, KERNEL_LIB)
+    for o in objs:
+        o.unlink()
+    return [ORACLE_LIB, KERNEL_LIB]
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", type=Path, default=Path("/root/reference"))
+    a = ap.parse_args(argv)
+    for p in build(a.ref):
+        print("built", p, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

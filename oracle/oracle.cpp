@@ -6,7 +6,14 @@
 #include "bse.hpp"
 #include "smallpt9_direct.hpp"
 
+#ifdef LOOM_ORACLE_REF
+// the reference's own kernels (oracle/_ref/liboracle_ref.so, built by build_ref.py)
+#include "ref_kernels.hpp"
+#define LOOM_ORACLE_CATALOG LOOM_REF_CATALOG
+#else
 #include "../loom_amd/kernels/catalog.hpp"
+#define LOOM_ORACLE_CATALOG LOOM_AMD_CATALOG
+#endif
 
 #include <algorithm>
 #include <atomic>
@@ -25,7 +32,7 @@ struct Entry
 };
 
 #define LOOM_ORACLE_ENTRY(NAME, ...) Entry{NAME, &build_program<__VA_ARGS__>},
-const Entry g_entries[] = {LOOM_AMD_CATALOG(LOOM_ORACLE_ENTRY)};
+const Entry g_entries[] = {LOOM_ORACLE_CATALOG(LOOM_ORACLE_ENTRY)};
 #undef LOOM_ORACLE_ENTRY
 
 constexpr int g_count = sizeof(g_entries) / sizeof(g_entries[0]);

@@ -9,17 +9,19 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
 PATH = ROOT / "oracle" / "_build" / "liboracle.so"
+# the reference's own kernels over the same interpreter (oracle/build_ref.py)
+REF_PATH = ROOT / "oracle" / "_ref" / "liboracle_ref.so"
 DTYPES = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64, 4: np.int8, 5: np.int16,
           6: np.int32, 7: np.int64, 8: np.float32, 9: np.float64}
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not PATH.exists():
-            raise RuntimeError(f"{PATH} missing: run `python -m loom_amd.build`")
-        L = C.CDLL(str(PATH))
+def lib(path=PATH):
+    path = Path(path)
+    if path not in _libs:
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `python -m loom_amd.build`")
+        L = C.CDLL(str(path))
         L.oracle_kernel_find.argtypes = [C.c_char_p]
         L.oracle_kernel_name.restype = C.c_char_p
         L.oracle_kernel_io.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -44,12 +46,12 @@ def lib():
         L.oracle_work.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint64, C.c_int,
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
-def io(name):
-    L = lib()
+def io(name, path=PATH):
+    L = lib(path)
     k = L.oracle_kernel_find(name.encode())
     if k < 0:
         raise KeyError(name)
@@ -59,16 +61,16 @@ def io(name):
     return k, [DTYPES[it[i]] for i in range(ni.value)], [DTYPES[ot[i]] for i in range(no.value)]
 
 
-def run(name, inputs, threads=8):
+def run(name, inputs, threads=8, path=PATH):
     """Outputs of kernel `name` on host columns `inputs` (oracle)."""
-    k, itys, otys = io(name)
+    k, itys, otys = io(name, path)
     ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
     n = len(ins[0])
     outs = [np.zeros(n, t) for t in otys]
     ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
     op = (C.c_void_p * 16)(*[a.ctypes.data for a in outs])
     ex, er = C.c_uint64(), C.c_uint32()
-    rc = lib().oracle_run(k, ip, op, 0, n, threads, C.byref(ex), C.byref(er))
+    rc = lib(path).oracle_run(k, ip, op, 0, n, threads, C.byref(ex), C.byref(er))
     assert rc == 0
     return outs, er.value
 
