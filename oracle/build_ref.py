@@ -1,8 +1,10 @@
 """ORACLE / TEST INFRASTRUCTURE: build the reference's own path-tracing
-kernels (oracle/ref_kernels.hpp) against this repository's front-end, from
-the reference checkout where it lies, into oracle/_ref/:
+kernels and test programs (oracle/ref_kernels.hpp) against this repository's
+front-end, from the reference checkout where it lies, into oracle/_ref/:
 
-* liboracle_ref.so — the oracle interpreter (oracle.cpp) over those kernels;
+* liboracle_ref.so — the oracle interpreter (oracle.cpp) over those kernels,
+  and the reference's own validators of the test programs
+  (ref_validators.cpp);
 * libloom_refkernels.so — the same kernels lowered for gfx950 by the
   product's backend; loading it (after the product library) registers them
   with the product library, where the C ABI finds them by name.
@@ -26,7 +28,8 @@ ROOT = HERE.parent
 OUT = HERE / "_ref"
 ORACLE_LIB = OUT / "liboracle_ref.so"
 KERNEL_LIB = OUT / "libloom_refkernels.so"
-SOURCES = ["ref_pt_const.hip", "ref_pt2_const.hip", "ref_pt2_const_1024.hip"]
+SOURCES = ["ref_pt_const.hip", "ref_pt2_const.hip", "ref_pt2_const_1024.hip",
+           "ref_programs_a.hip", "ref_programs_b.hip", "ref_programs_c.hip"]
 
 
 def _run(cmd):
@@ -39,6 +42,9 @@ def build(ref: Path) -> list[Path]:
     sys.path.insert(0, str(ROOT))
     from loom_amd import build as B
     src = ref / "src"
+    # static_math: the constexpr math library the reference vendors for its
+    # common/math.hpp (loom::fac, used by the validators)
+    ext = ref / "ext" / "include"
     if not (src / "smallpt" / "vm" / "pt.hpp").exists():
         return []
     OUT.mkdir(parents=True, exist_ok=True)
@@ -47,18 +53,18 @@ def build(ref: Path) -> list[Path]:
         tmp = ORACLE_LIB.with_suffix(".tmp")
         _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",
               "-fno-fast-math", "-DLOOM_ORACLE_REF", f"-I{ROOT / 'include' / 'loom_vm'}",
-              f"-I{ROOT / 'include'}", f"-I{HERE}", f"-I{src}", "-shared", "-o", str(tmp),
-              str(HERE / "oracle.cpp"), "-lpthread"])
+              f"-I{ROOT / 'include'}", f"-I{HERE}", f"-I{src}", f"-I{ext}", "-shared", "-o",
+              str(tmp), str(HERE / "oracle.cpp"), str(HERE / "ref_validators.cpp"), "-lpthread"])
         os.replace(tmp, ORACLE_LIB)
         return ORACLE_LIB
 
     def obj(name):
         o = OUT / (Path(name).stem + ".o")
         _run([B.HIPCC, *B.COMMON, f"-I{ROOT / 'loom_amd' / 'csrc'}", f"-I{HERE}", f"-I{src}",
-              "-x", "hip", "-c", str(HERE / name), "-o", str(o)])
+              f"-I{ext}", "-x", "hip", "-c", str(HERE / name), "-o", str(o)])
         return o
 
-    with cf.ThreadPoolExecutor(len(SOURCES) + 1) as ex:
+    with cf.ThreadPoolExecutor(min(len(SOURCES) + 1, 8)) as ex:
         fo = ex.submit(oracle)
         objs = list(ex.map(obj, SOURCES))
         fo.result()

@@ -8,11 +8,18 @@
 /// (src/benchmarks/pt.cpp:31-35, CMakeLists.txt:419-460): 40x40 pixels, 64
 /// samples at least, blur / threshold / maximum depth as in the names; and
 /// the second one on a 1024x1024 frame.
+/// Also the reference's own test programs (src/tests/programs.hpp), at the
+/// element types its VM test suites instantiate them with
+/// (src/tests/vm.cpp:262-325 for the scalar and vector engines,
+/// src/tests/vm_simd.cpp:71-90 and vm_simd_compact.cpp:180-251 for the SIMD
+/// and compact engines, src/tests/vm.cu:31-41 for the CUDA backend).
 /// Built only by oracle/build_ref.py, into oracle/_ref/, where the reference
 /// sources exist.
 #pragma once
 
+#include <common/math.hpp> // loom::fac, for the validators of programs.hpp
 #include <smallpt/vm/pt.hpp>
+#include <tests/programs.hpp>
 
 namespace loom_ref
 {
@@ -33,5 +40,55 @@ using c_b5_t0005_d20_1024 = loom::vm::constants<CS(1024), loom::sl2cs<"0.0005">,
 #define LOOM_REF_CATALOG_PT2_CONST_1024(X) \
     X("ref_pt2_const_b5_t0.0005_d20_1024", \
       loom::vm::pt2_const<float, loom_ref::c_b5_t0005_d20_1024>)
-#define LOOM_REF_CATALOG(X) \
-    LOOM_REF_CATALOG_PT_CONST(X) LOOM_REF_CATALOG_PT2_CONST(X) LOOM_REF_CATALOG_PT2_CONST_1024(X)
+
+/// The test programs, grouped so that each group is one compilation unit.
+#define LOOM_REF_EACH_INT(X, NAME, K)                                          \
+    X("ref_" NAME "_i8", K<std::int8_t>) X("ref_" NAME "_i16", K<std::int16_t>) \
+    X("ref_" NAME "_i32", K<std::int32_t>) X("ref_" NAME "_i64", K<std::int64_t>)
+#define LOOM_REF_EACH_UINT(X, NAME, K)                                           \
+    X("ref_" NAME "_u8", K<std::uint8_t>) X("ref_" NAME "_u16", K<std::uint16_t>) \
+    X("ref_" NAME "_u32", K<std::uint32_t>) X("ref_" NAME "_u64", K<std::uint64_t>)
+#define LOOM_REF_EACH_FLOAT(X, NAME, K) \
+    X("ref_" NAME "_f32", K<float>) X("ref_" NAME "_f64", K<double>)
+
+#define LOOM_REF_CATALOG_PROGRAMS_A(X)                                     \
+    LOOM_REF_EACH_INT(X, "increment", increment)                           \
+    LOOM_REF_EACH_UINT(X, "increment", increment)                          \
+    X("ref_fac_test_u32", fac_test<std::uint32_t>)                         \
+    X("ref_fac_test_u64", fac_test<std::uint64_t>)                         \
+    X("ref_fib_test_u64", fib_test<std::uint64_t>)                         \
+    LOOM_REF_EACH_FLOAT(X, "union_test", union_test)                       \
+    LOOM_REF_EACH_FLOAT(X, "union_cmov_test", union_cmov_test)             \
+    LOOM_REF_EACH_UINT(X, "jmp4_test", jmp4_test)                          \
+    X("ref_jmp4_test_f32", jmp4_test<float>)                               \
+    X("ref_lshift_test_u16", lshift_test<std::uint16_t>)                   \
+    X("ref_lshift_test_u32", lshift_test<std::uint32_t>)                   \
+    X("ref_lshift_test_u64", lshift_test<std::uint64_t>)                   \
+    X("ref_rshift_test_u16", rshift_test<std::uint16_t>)                   \
+    X("ref_rshift_test_u32", rshift_test<std::uint32_t>)                   \
+    X("ref_rshift_test_u64", rshift_test<std::uint64_t>)
+#define LOOM_REF_CATALOG_PROGRAMS_B(X)                                     \
+    LOOM_REF_EACH_UINT(X, "bit_and_test", bit_and_test)                    \
+    LOOM_REF_EACH_UINT(X, "bit_or_test", bit_or_test)                      \
+    LOOM_REF_EACH_UINT(X, "bit_xor_test", bit_xor_test)                    \
+    LOOM_REF_EACH_UINT(X, "bit_not_test", bit_not_test)                    \
+    LOOM_REF_EACH_UINT(X, "bool_and_test", bool_and_test)                  \
+    LOOM_REF_EACH_UINT(X, "bool_or_test", bool_or_test)                    \
+    LOOM_REF_EACH_UINT(X, "bool_not_test", bool_not_test)                  \
+    LOOM_REF_EACH_UINT(X, "sct_inv_test", sct_inv_test)                    \
+    LOOM_REF_EACH_UINT(X, "slt_inv_test", slt_inv_test)
+#define LOOM_REF_CATALOG_PROGRAMS_C(X)                                     \
+    LOOM_REF_EACH_FLOAT(X, "abs_test", abs_test)                           \
+    LOOM_REF_EACH_FLOAT(X, "trunc_test", trunc_test)                       \
+    LOOM_REF_EACH_FLOAT(X, "ceilfloor", ceilfloor)                         \
+    LOOM_REF_EACH_FLOAT(X, "minmax_test", minmax_test)                     \
+    LOOM_REF_EACH_FLOAT(X, "arithmetic_test", arithmetic_test)             \
+    LOOM_REF_EACH_FLOAT(X, "sin_cos_test", sin_cos_test)                   \
+    LOOM_REF_EACH_FLOAT(X, "sqrt_test", sqrt_test)                         \
+    LOOM_REF_EACH_FLOAT(X, "rsqrt_test", rsqrt_test)
+#define LOOM_REF_CATALOG_PROGRAMS(X) \
+    LOOM_REF_CATALOG_PROGRAMS_A(X) LOOM_REF_CATALOG_PROGRAMS_B(X) LOOM_REF_CATALOG_PROGRAMS_C(X)
+
+#define LOOM_REF_CATALOG(X)                                                                       \
+    LOOM_REF_CATALOG_PT_CONST(X) LOOM_REF_CATALOG_PT2_CONST(X) LOOM_REF_CATALOG_PT2_CONST_1024(X) \
+    LOOM_REF_CATALOG_PROGRAMS(X)

@@ -1,0 +1,49 @@
+"""The reference's own test programs (src/tests/programs.hpp) on the GPU
+through the C ABI, in every form, bit-exact against the oracle interpreter
+running the same programs and against the reference's own validators — the
+checks its CUDA backend test makes (src/tests/vm.cu:58-86: function and
+stream forms against fn::val_*), over every instantiation its VM suites use."""
+import numpy as np
+import pytest
+
+import loom_amd
+from loom_amd import abi
+from tests import oracle_lib as O
+from tests import refprograms as P
+from tests.test_refkernels import REF_LIB, load_ref_kernels
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not (REF_LIB.exists() and O.REF_PATH.exists()),
+                                 reason="reference kernels not built (no reference checkout)")]
+
+# 2^16 threads: with one wave per CU the hybrid form holds 16384 lanes, so
+# every lane takes about four threads from the stream in turn
+N = 1 << 16
+FORMS = [
+    ("compact_sm", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR)),
+    ("compact_topo", abi.MODE_COMPACT, dict(reconv=abi.RECONV_TOPO)),
+    ("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+    ("hybrid_sweep", abi.MODE_HYBRID, {}),
+    ("hybrid_stream_1wpc", abi.MODE_HYBRID, dict(waves_per_cu=1, live_threshold=0)),
+    ("hybrid_sm_regroup", abi.MODE_HYBRID, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR,
+                                                regroup=True, live_threshold=32)),
+]
+
+
+@pytest.mark.parametrize("name", P.NAMES)
+def test_reference_program_parity(gpu, name):
+    load_ref_kernels()
+    cols = P.inputs(name, N, seed=1)
+    (want,), err = O.run(name, cols, threads=16, path=O.REF_PATH)
+    assert err == 0
+    expect = P.validate(name, cols)
+    if expect is not None:
+        assert np.array_equal(want, expect), name
+    k = loom_amd.kernel(name)
+    for fname, mode, params in FORMS:
+        print(f"[gpu] {name} {fname} n={N}", flush=True)
+        r = loom_amd.run(k, mode, cols, params=loom_amd.make_params(mode=mode, **params))
+        assert r.errors == 0, (name, fname)
+        (got,) = r.outputs
+        assert got.dtype == want.dtype
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (name, fname)
