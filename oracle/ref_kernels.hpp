@@ -12,12 +12,17 @@
 /// element types its VM test suites instantiate them with
 /// (src/tests/vm.cpp:262-325 for the scalar and vector engines,
 /// src/tests/vm_simd.cpp:71-90 and vm_simd_compact.cpp:180-251 for the SIMD
-/// and compact engines, src/tests/vm.cu:31-41 for the CUDA backend).
+/// and compact engines, src/tests/vm.cu:31-41 for the CUDA backend), and
+/// the floating-point benchmark kernel fp_mix_k (src/benchmarks/fp_mix.hpp:
+/// 95-119; tested by src/tests/vm_fp_mix.cpp:33-41).
 /// Built only by oracle/build_ref.py, into oracle/_ref/, where the reference
 /// sources exist.
 #pragma once
 
-#include <common/math.hpp> // loom::fac, for the validators of programs.hpp
+// common/math.hpp first: it declares loom::fac and loom::fma, which the
+// validators in programs.hpp and fp_mix.hpp call
+#include <common/math.hpp>
+#include <benchmarks/fp_mix.hpp>
 #include <smallpt/vm/pt.hpp>
 #include <tests/programs.hpp>
 
@@ -85,7 +90,8 @@ using c_b5_t0005_d20_1024 = loom::vm::constants<CS(1024), loom::sl2cs<"0.0005">,
     LOOM_REF_EACH_FLOAT(X, "arithmetic_test", arithmetic_test)             \
     LOOM_REF_EACH_FLOAT(X, "sin_cos_test", sin_cos_test)                   \
     LOOM_REF_EACH_FLOAT(X, "sqrt_test", sqrt_test)                         \
-    LOOM_REF_EACH_FLOAT(X, "rsqrt_test", rsqrt_test)
+    LOOM_REF_EACH_FLOAT(X, "rsqrt_test", rsqrt_test)                       \
+    LOOM_REF_EACH_FLOAT(X, "fp_mix_k", fp_mix_k)
 #define LOOM_REF_CATALOG_PROGRAMS(X) \
     LOOM_REF_CATALOG_PROGRAMS_A(X) LOOM_REF_CATALOG_PROGRAMS_B(X) LOOM_REF_CATALOG_PROGRAMS_C(X)
 

@@ -10,10 +10,16 @@
 /// reference's suites compare their engines with these functions
 /// (src/tests/vm.cpp:241-246, vm.cu:58-82).
 ///
+/// fp_mix_k (src/benchmarks/fp_mix.hpp) has its reference fp_mix_ref, which
+/// the reference compares within 100 ULP on 99% of inputs
+/// (src/tests/vm_test_suite_utils.hpp:150-182); ref_ulp_mismatches applies
+/// the reference's almost_equal (src/common/math.hpp:253-274) for that.
+///
 /// The programs checked by an invariant instead (abs_test ... rsqrt_test,
 /// sct_inv_test, slt_inv_test: every output non-zero,
 /// src/common/test.hpp:636-648) have no validator here.
-#include <common/math.hpp>
+#include <common/math.hpp> // before fp_mix.hpp, whose fp_mix_ref calls loom::fma
+#include <benchmarks/fp_mix.hpp>
 #include <tests/programs.hpp>
 
 #include <cstdint>
@@ -91,7 +97,18 @@ const Entry kEntries[] = {
     REF_V_UINT2("bool_and_test", std::logical_and<>{})
     REF_V_UINT2("bool_or_test", std::logical_or<>{})
     REF_V_UINT1("bool_not_test", std::logical_not<>{})
+    REF_V("ref_fp_mix_k_f32", float, fp_mix_ref{}, float, float, float)
+    REF_V("ref_fp_mix_k_f64", double, fp_mix_ref{}, double, double, double)
 };
+
+template <typename T>
+std::uint64_t
+ulp_mismatches(const T* a, const T* b, std::uint64_t n, unsigned ulp)
+{
+    std::uint64_t bad = 0;
+    for(std::uint64_t t = 0; t < n; ++t) bad += !loom::almost_equal(a[t], b[t], ulp);
+    return bad;
+}
 } // namespace
 
 /// Applies the reference's validator for program @p name to @p n threads of
@@ -107,4 +124,14 @@ ref_validate(const char* name, const void* const* in, void* out, std::uint64_t n
             return 0;
         }
     return -1;
+}
+
+/// Number of the @p n pairs a[t], b[t] that the reference's
+/// loom::almost_equal(a, b, ulp) rejects; @p width 4 for float, 8 for double.
+extern "C" std::uint64_t
+ref_ulp_mismatches(const void* a, const void* b, std::uint64_t n, unsigned ulp, int width)
+{
+    if(width == 4)
+        return ulp_mismatches(static_cast<const float*>(a), static_cast<const float*>(b), n, ulp);
+    return ulp_mismatches(static_cast<const double*>(a), static_cast<const double*>(b), n, ulp);
 }

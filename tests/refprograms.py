@@ -11,7 +11,8 @@ import numpy as np
 from tests import oracle_lib as O
 
 # programs.hpp kernel -> (input generator, checked by: "validator" = the
-# reference's fn::val_* / std functor, "invariant" = every output non-zero)
+# reference's fn::val_* / std functor, "invariant" = every output non-zero,
+# "fuzzy" = its reference function within 100 ULP on 99% of inputs)
 # Generators follow src/tests/generators.hpp:30-210 and the suites that pick
 # them (src/tests/vm.cpp:262-325, vm_simd_compact.cpp:180-251).
 PROGRAMS = {
@@ -40,6 +41,9 @@ PROGRAMS = {
     "sin_cos_test": ("trig", "invariant"),    # GMTrig: [-pi, pi]
     "sqrt_test": ("positive", "invariant"),   # GMFloatPositive: [1e-6, 1e6]
     "rsqrt_test": ("positive", "invariant"),
+    # src/benchmarks/fp_mix.hpp against its fp_mix_ref within 100 ULP on 99%
+    # of GM_F3 inputs (src/tests/vm_fp_mix.cpp:33-41)
+    "fp_mix_k": ("float", "fuzzy"),
 }
 TYPES = {"i8": np.int8, "i16": np.int16, "i32": np.int32, "i64": np.int64, "u8": np.uint8,
          "u16": np.uint16, "u32": np.uint32, "u64": np.uint64, "f32": np.float32,
@@ -56,7 +60,7 @@ NAMES = ([f"ref_increment_{t}" for t in ("i8", "i16", "i32", "i64", "u8", "u16",
             for t in ("u8", "u16", "u32", "u64")]
          + [f"ref_{p}_{t}" for p in ("abs_test", "trunc_test", "ceilfloor", "minmax_test",
                                      "arithmetic_test", "sin_cos_test", "sqrt_test",
-                                     "rsqrt_test") for t in ("f32", "f64")])
+                                     "rsqrt_test", "fp_mix_k") for t in ("f32", "f64")])
 
 
 def split(name):
@@ -113,7 +117,7 @@ def validate(name, cols):
     """The reference validator's outputs for `cols`, or None for a program
     checked by an invariant."""
     body, _ = split(name)
-    if PROGRAMS[body][1] != "validator":
+    if PROGRAMS[body][1] == "invariant":
         return None
     L = O.lib(O.REF_PATH)
     L.ref_validate.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p, C.c_uint64]
@@ -124,3 +128,13 @@ def validate(name, cols):
     ip = (C.c_void_p * 16)(*[c.ctypes.data for c in cols])
     assert L.ref_validate(name.encode(), ip, out.ctypes.data, len(out)) == 0, name
     return out
+
+
+def ulp_mismatches(a, b, ulp=100):
+    """How many pairs the reference's loom::almost_equal(a, b, ulp) rejects."""
+    L = O.lib(O.REF_PATH)
+    L.ref_ulp_mismatches.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint, C.c_int]
+    L.ref_ulp_mismatches.restype = C.c_uint64
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    assert a.dtype == b.dtype and a.dtype.kind == "f" and len(a) == len(b)
+    return L.ref_ulp_mismatches(a.ctypes.data, b.ctypes.data, len(a), ulp, a.dtype.itemsize)

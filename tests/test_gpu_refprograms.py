@@ -37,7 +37,9 @@ def test_reference_program_parity(gpu, name):
     (want,), err = O.run(name, cols, threads=16, path=O.REF_PATH)
     assert err == 0
     expect = P.validate(name, cols)
-    if expect is not None:
+    if expect is not None and P.PROGRAMS[P.split(name)[0]][1] == "fuzzy":
+        assert P.ulp_mismatches(want, expect) <= 0.01 * N, name
+    elif expect is not None:
         assert np.array_equal(want, expect), name
     k = loom_amd.kernel(name)
     for fname, mode, params in FORMS:
