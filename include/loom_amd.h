@@ -144,7 +144,11 @@ int loom_kernel_find(const char* name);
  * terminating NUL; *needed receives the full size.  Needs no device. */
 int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
 /* VGPRs and scratch bytes per lane of a compiled kernel (mode: compact or
- * hybrid pass kernel; policy: loom_params.reconv). */
+ * hybrid pass kernel; policy: loom_params.reconv, or'ed with
+ * LOOM_RESOURCES_REGROUP for the hybrid pass kernel that regroups, which a
+ * run with loom_params.regroup set launches; runs without it launch a pass
+ * kernel compiled without the queue paths). */
+#define LOOM_RESOURCES_REGROUP 0x100
 int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
 /* Register r (0 <= r < n_registers) of a kernel: its name (NUL-terminated,
  * at most len bytes), first type (loom_type), width in bytes, number of

@@ -17,6 +17,8 @@ NUM_COUNTERS = 16
 
 MODE_COMPACT = 0
 MODE_HYBRID = 1
+# or'ed into loom_kernel_resources' policy: the pass kernel that regroups
+RESOURCES_REGROUP = 0x100
 
 RECONV_SUCCESSOR_MAJOR = 0
 RECONV_TOPO = 1

@@ -1020,7 +1020,14 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 /// sized to the resident wave count; every wave loops claiming batches until
 /// the pass has none left, so each wave exits after finite work.  Waves
 /// beyond the pass's useful work exit at once.
-template <typename K, int P>
+///
+/// @p RG compiles the regroup paths (queue fills and spills, a switch over
+/// every label that reads or writes the registers live there).  Without
+/// them the register file is not held live across those paths and the
+/// kernel needs far fewer VGPRs for a large register file (the reference's
+/// pt2_const: 239 -> 138, two -> three waves per SIMD), so runs that do not
+/// regroup launch the RG = false kernel, in which no queue is ever read.
+template <typename K, int P, bool RG>
 __global__ __launch_bounds__(64) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
@@ -1044,9 +1051,9 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
     // pass does; survivors finish in place (no pass per remaining block)
     const bool drain = left == 0 && nbq <= gridDim.x;
 
-    const char*    qin  = h.q[par];
-    char*          qout = h.q[par ^ 1u];
-    std::uint32_t* qcnt = ctl->qcount[par ^ 1u];
+    [[maybe_unused]] const char*    qin  = h.q[par];
+    [[maybe_unused]] char*          qout = h.q[par ^ 1u];
+    [[maybe_unused]] std::uint32_t* qcnt = ctl->qcount[par ^ 1u];
 
     WaveCtx w(c);
     std::uint64_t spill_rows = 0, spill_bytes = 0, fill_rows = 0, fill_bytes = 0,
@@ -1065,25 +1072,28 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
         pc              = HALT;
         has             = false;
         std::uint32_t b = nbq;
-        if(lane == 0 && nbq) b = atomicAdd(&ctl->batch_next, 1u);
+        if(RG && lane == 0 && nbq) b = atomicAdd(&ctl->batch_next, 1u);
         b = uniform(b);
-        if(b < nbq)
+        if(RG && b < nbq)
         {
-            // label of batch b: last label whose prefix is <= b
-            std::uint32_t label = 0;
-            while(label + 1 < 64 && ctl->bprefix[label + 1] <= b) ++label;
-            label                    = uniform(label);
-            const std::uint32_t cnt  = uniform(ctl->qcount[par][label]);
-            const std::uint32_t row0 = (b - uniform(ctl->bprefix[label])) * 64u;
-            const std::uint32_t rows = cnt - row0 < 64u ? cnt - row0 : 64u;
-            if(lane < rows)
+            if constexpr(RG)
             {
-                tid = fill_label<K>(label, r, qin, h.qcap, row0 + lane);
-                pc  = label;
-                has = true;
+                // label of batch b: last label whose prefix is <= b
+                std::uint32_t label = 0;
+                while(label + 1 < 64 && ctl->bprefix[label + 1] <= b) ++label;
+                label                    = uniform(label);
+                const std::uint32_t cnt  = uniform(ctl->qcount[par][label]);
+                const std::uint32_t row0 = (b - uniform(ctl->bprefix[label])) * 64u;
+                const std::uint32_t rows = cnt - row0 < 64u ? cnt - row0 : 64u;
+                if(lane < rows)
+                {
+                    tid = fill_label<K>(label, r, qin, h.qcap, row0 + lane);
+                    pc  = label;
+                    has = true;
+                }
+                fill_rows += rows;
+                fill_bytes += std::uint64_t(rows) * label_bytes<K>(label);
             }
-            fill_rows += rows;
-            fill_bytes += std::uint64_t(rows) * label_bytes<K>(label);
         }
         else
         {
@@ -1142,35 +1152,38 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                     continue;
                 }
             }
-            if(c.regroup && !drain)
+            if constexpr(RG)
             {
-                // spill survivors into the queues of their labels
-                std::uint64_t todo = live;
-                while(todo)
+                if(c.regroup && !drain)
                 {
-                    const int           l     = __builtin_ctzll(todo);
-                    const std::uint32_t label = uniform(__builtin_amdgcn_readlane(pc, l));
-                    const std::uint64_t m     = ballot(has && pc == label);
-                    todo &= ~m;
-                    const std::uint32_t cntm = popc(m);
-                    std::uint32_t       base = 0;
-                    if(lane == 0) base = atomicAdd(&qcnt[label], cntm);
-                    base = uniform(base);
-                    const std::uint32_t row = base + rank_in(m);
-                    if(has && pc == label && row < h.qcap)
+                    // spill survivors into the queues of their labels
+                    std::uint64_t todo = live;
+                    while(todo)
                     {
-                        spill_label<K>(label, r, tid, qout, h.qcap, row);
-                        has = false;
-                        pc  = HALT;
+                        const int           l     = __builtin_ctzll(todo);
+                        const std::uint32_t label = uniform(__builtin_amdgcn_readlane(pc, l));
+                        const std::uint64_t m     = ballot(has && pc == label);
+                        todo &= ~m;
+                        const std::uint32_t cntm = popc(m);
+                        std::uint32_t       base = 0;
+                        if(lane == 0) base = atomicAdd(&qcnt[label], cntm);
+                        base = uniform(base);
+                        const std::uint32_t row = base + rank_in(m);
+                        if(has && pc == label && row < h.qcap)
+                        {
+                            spill_label<K>(label, r, tid, qout, h.qcap, row);
+                            has = false;
+                            pc  = HALT;
+                        }
+                        const std::uint32_t ok =
+                            base >= h.qcap ? 0u :
+                                             (h.qcap - base < cntm ? std::uint32_t(h.qcap - base) : cntm);
+                        spill_rows += ok;
+                        spill_bytes += std::uint64_t(ok) * label_bytes<K>(label);
+                        spill_failed += cntm - ok;
                     }
-                    const std::uint32_t ok =
-                        base >= h.qcap ? 0u :
-                                         (h.qcap - base < cntm ? std::uint32_t(h.qcap - base) : cntm);
-                    spill_rows += ok;
-                    spill_bytes += std::uint64_t(ok) * label_bytes<K>(label);
-                    spill_failed += cntm - ok;
+                    if(!ballot(has)) break;
                 }
-                if(!ballot(has)) break;
             }
             // no regroup (or a full queue): finish the remaining lanes in place
             finish = true;

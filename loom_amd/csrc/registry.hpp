@@ -32,9 +32,10 @@ struct KernelEntry
     hipError_t (*hybrid_pass[kNumPolicies])(const IoArgs&, const CtlArgs&, const HybridArgs&,
                                  unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
-    int (*hybrid_occupancy[kNumPolicies])()                            = {};
+    /// (with or without the regroup paths)
+    int (*hybrid_occupancy[kNumPolicies])(bool regroup)                = {};
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
-    int (*kernel_attrs)(int mode, int policy, int* vgprs, int* scratch) = nullptr;
+    int (*kernel_attrs)(int mode, int policy, bool regroup, int* vgprs, int* scratch) = nullptr;
     std::string (*dump)() = nullptr;
     /// step trace of threads [0, n) (trace_kernel)
     hipError_t (*trace)(const IoArgs&, std::uint64_t n, std::uint32_t max_steps,
@@ -110,38 +111,50 @@ launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
     return hipGetLastError();
 }
 
+/// Runs that regroup launch the pass kernel with the queue paths compiled
+/// in; the others the kernel without them.
 template <typename K, int P>
 hipError_t
 launch_hybrid_pass(const IoArgs& io, const CtlArgs& c, const HybridArgs& h, unsigned grid,
                    hipStream_t s)
 {
-    hipLaunchKernelGGL((hybrid_pass_kernel<K, P>), dim3(grid), dim3(64), 0, s, io, c, h);
+    if(c.regroup)
+        hipLaunchKernelGGL((hybrid_pass_kernel<K, P, true>), dim3(grid), dim3(64), 0, s, io, c, h);
+    else
+        hipLaunchKernelGGL((hybrid_pass_kernel<K, P, false>), dim3(grid), dim3(64), 0, s, io, c, h);
     return hipGetLastError();
 }
 
 template <typename K, int P>
 int
-hybrid_occupancy()
+hybrid_occupancy(bool regroup)
 {
     int nblk = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K, P>, 64, 0) !=
-       hipSuccess)
-        return 1;
+    const hipError_t e =
+        regroup ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K, P, true>,
+                                                               64, 0) :
+                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, hybrid_pass_kernel<K, P, false>,
+                                                               64, 0);
+    if(e != hipSuccess) return 1;
     return nblk > 0 ? nblk : 1;
 }
 
 /// @brief Registers and scratch of the compiled kernel of @p mode and
 /// @p policy (LOOM_MODE_COMPACT: the predicated kernel; LOOM_MODE_HYBRID: the
-/// pass kernel).
+/// pass kernel, with the regroup paths when @p regroup).
 template <typename K>
 int
-kernel_attrs(int mode, int policy, int* vgprs, int* scratch)
+kernel_attrs(int mode, int policy, bool regroup, int* vgprs, int* scratch)
 {
     const void* f = nullptr;
     auto        pick = [&]<int P>()
     {
-        f = mode == LOOM_MODE_COMPACT ? reinterpret_cast<const void*>(compact_kernel<K, P>) :
-                                        reinterpret_cast<const void*>(hybrid_pass_kernel<K, P>);
+        if(mode == LOOM_MODE_COMPACT)
+            f = reinterpret_cast<const void*>(compact_kernel<K, P>);
+        else if(regroup)
+            f = reinterpret_cast<const void*>(hybrid_pass_kernel<K, P, true>);
+        else
+            f = reinterpret_cast<const void*>(hybrid_pass_kernel<K, P, false>);
     };
     if(policy == POLICY_SUCCESSOR_MAJOR) pick.template operator()<POLICY_SUCCESSOR_MAJOR>();
     else if(policy == POLICY_TOPO) pick.template operator()<POLICY_TOPO>();

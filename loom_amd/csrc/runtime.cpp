@@ -217,7 +217,7 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
 {
     // persistent grid: resident 64-lane workgroups
     const int policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
-    const int occ    = e.hybrid_occupancy[policy]();
+    const int occ    = e.hybrid_occupancy[policy](c.regroup != 0);
     int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
     wpc           = std::max(1, std::min(wpc, occ));
     unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
@@ -520,8 +520,10 @@ loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch
     if(int r = check_kernel(kernel)) return r;
     if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
         return fail(LOOM_E_ARG, "unknown mode %d", mode);
+    const bool regroup = (policy & LOOM_RESOURCES_REGROUP) != 0;
+    policy &= ~LOOM_RESOURCES_REGROUP;
     if(policy < 0 || policy >= kNumPolicies) return fail(LOOM_E_ARG, "unknown policy %d", policy);
-    if(int r = registry()[kernel].kernel_attrs(mode, policy, vgprs, scratch_bytes))
+    if(int r = registry()[kernel].kernel_attrs(mode, policy, regroup, vgprs, scratch_bytes))
         return fail(r, "hipFuncGetAttributes failed");
     return 0;
 }

@@ -145,10 +145,12 @@ def resources(name) -> dict:
     loom_amd.init()
     k = loom_amd.kernel(name)
     out = {}
-    for mode, mname in ((abi.MODE_COMPACT, "compact"), (abi.MODE_HYBRID, "hybrid")):
+    forms = ((abi.MODE_COMPACT, "compact", 0), (abi.MODE_HYBRID, "hybrid", 0),
+             (abi.MODE_HYBRID, "hybrid-regroup", abi.RESOURCES_REGROUP))
+    for mode, mname, flag in forms:
         for pol, pname in ((0, "successor-major"), (1, "topo"), (2, "sweep")):
             v, sc = C.c_int(), C.c_int()
-            abi.check(abi.load().loom_kernel_resources(k.index, mode, pol, C.byref(v),
+            abi.check(abi.load().loom_kernel_resources(k.index, mode, pol | flag, C.byref(v),
                                                        C.byref(sc)), "loom_kernel_resources")
             out[f"{mname}/{pname}"] = {"vgprs": v.value, "scratch_bytes": sc.value}
     return out

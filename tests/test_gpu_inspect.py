@@ -27,3 +27,15 @@ def test_inspector_counter_run(gpu):
         assert c[form]["active_ops"] <= c[form]["slot_ops"]
     # the same threads run the same blocks in both forms
     assert c["compact"]["thread_steps"] == c["hybrid"]["thread_steps"]
+
+
+def test_pass_kernel_without_regroup_is_not_larger(gpu):
+    """Runs that do not regroup launch a pass kernel without the queue paths;
+    it never needs more registers than the one that regroups."""
+    print("[gpu] pass kernel resources with and without regroup", flush=True)
+    for name in ("smallpt9_f32", "collatz_u64", "fp_mix_f32"):
+        res = inspect.resources(name)
+        for pol in ("successor-major", "topo", "sweep"):
+            plain, rg = res[f"hybrid/{pol}"], res[f"hybrid-regroup/{pol}"]
+            print(f"[gpu] {name} {pol}: {plain['vgprs']} / {rg['vgprs']} VGPRs", flush=True)
+            assert plain["vgprs"] <= rg["vgprs"], (name, pol)
