@@ -37,6 +37,30 @@ using c_b5_t0005_d20_1024 = loom::vm::constants<CS(1024), loom::sl2cs<"0.0005">,
                                                 loom::sl2cs<"5">, loom::sl2cs<"20">>;
 } // namespace loom_ref
 
+namespace loom_ref
+{
+// clang-format off
+/// The reference's random-number IR macros (src/vm/lib/random.hpp:30-85) in
+/// the order its path tracer applies them (src/smallpt/vm/pt.hpp:890-895,
+/// 277-278): the Threefry-2x32 block of counter (c0, c1) under the key its
+/// benchmarks use (0xdead, 0xbeef), the counter bumped by 39916801, and both
+/// block words as canonical floats.  Outputs: the block words, the bumped
+/// counter, the two floats.
+template <std::floating_point T>
+KERNEL(rng_macros, IVAR(c0), IVAR(c1))
+    IMOV(u0, c0),
+    IMOV(u1, c1),
+    INSERT_MACRO(loom::vm::threefry2x32r32, CS(u0), CS(u1), CS(0xdead), CS(0xbeef), CS(0x1bd17b98),
+                 CS(tmp0), CS(tmp1)),
+    INSERT_MACRO(loom::vm::bump_ctr, CS(c0), CS(c1), CS(39916801), CS(tmp0), CS(tmp1), CS(tmp2)),
+    IMOV(w0, u0),
+    IMOV(w1, u1),
+    INSERT_TMACRO(loom::vm::gen_canonical, CS(u0)),
+    INSERT_TMACRO(loom::vm::gen_canonical, CS(u1))
+KERNEL_END(IVAR(w0), IVAR(w1), IVAR(c0), IVAR(c1), TVAR(u0), TVAR(u1));
+// clang-format on
+} // namespace loom_ref
+
 /// X(name, kernel type) for every reference kernel of the check.
 #define LOOM_REF_CATALOG_PT_CONST(X) \
     X("ref_pt_const_b1_t0.0001_d2", loom::vm::pt_const<float, loom_ref::c_b1_t0001_d2>)
@@ -81,7 +105,8 @@ using c_b5_t0005_d20_1024 = loom::vm::constants<CS(1024), loom::sl2cs<"0.0005">,
     LOOM_REF_EACH_UINT(X, "bool_or_test", bool_or_test)                    \
     LOOM_REF_EACH_UINT(X, "bool_not_test", bool_not_test)                  \
     LOOM_REF_EACH_UINT(X, "sct_inv_test", sct_inv_test)                    \
-    LOOM_REF_EACH_UINT(X, "slt_inv_test", slt_inv_test)
+    LOOM_REF_EACH_UINT(X, "slt_inv_test", slt_inv_test)                    \
+    X("ref_rng_macros_f32", loom_ref::rng_macros<float>)
 #define LOOM_REF_CATALOG_PROGRAMS_C(X)                                     \
     LOOM_REF_EACH_FLOAT(X, "abs_test", abs_test)                           \
     LOOM_REF_EACH_FLOAT(X, "trunc_test", trunc_test)                       \

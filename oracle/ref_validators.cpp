@@ -15,11 +15,16 @@
 /// (src/tests/vm_test_suite_utils.hpp:150-182); ref_ulp_mismatches applies
 /// the reference's almost_equal (src/common/math.hpp:253-274) for that.
 ///
+/// ref_rng applies the reference's C++ Threefry-2x32, bump_ctr and
+/// gen_canonical (src/common/random.hpp:37-103), the functions its own test
+/// holds the IR macros to (src/tests/vm_smallpt.cpp:358-376).
+///
 /// The programs checked by an invariant instead (abs_test ... rsqrt_test,
 /// sct_inv_test, slt_inv_test: every output non-zero,
 /// src/common/test.hpp:636-648) have no validator here.
 #include <common/math.hpp> // before fp_mix.hpp, whose fp_mix_ref calls loom::fma
 #include <benchmarks/fp_mix.hpp>
+#include <common/random.hpp>
 #include <tests/programs.hpp>
 
 #include <cstdint>
@@ -134,4 +139,26 @@ ref_ulp_mismatches(const void* a, const void* b, std::uint64_t n, unsigned ulp, 
     if(width == 4)
         return ulp_mismatches(static_cast<const float*>(a), static_cast<const float*>(b), n, ulp);
     return ulp_mismatches(static_cast<const double*>(a), static_cast<const double*>(b), n, ulp);
+}
+
+/// The reference's C++ random-number functions on counters (c0[t], c1[t]):
+/// the Threefry-2x32 block under key (0xdead, 0xbeef) into (w0, w1), the
+/// counter bumped by 39916801 into (n0, n1), and gen_canonical of the block
+/// words into (f0, f1).
+extern "C" void
+ref_rng(const std::uint32_t* c0, const std::uint32_t* c1, std::uint32_t* w0, std::uint32_t* w1,
+        std::uint32_t* n0, std::uint32_t* n1, float* f0, float* f1, std::uint64_t n)
+{
+    for(std::uint64_t t = 0; t < n; ++t)
+    {
+        const loom::fn::ar_32_2 c{c0[t], c1[t]};
+        const loom::fn::ar_32_2 w = loom::threefry2x32r32<0xdead, 0xbeef>(c);
+        const loom::fn::ar_32_2 b = loom::bump_ctr(c, 39916801u);
+        w0[t]                     = w[0];
+        w1[t]                     = w[1];
+        n0[t]                     = b[0];
+        n1[t]                     = b[1];
+        f0[t]                     = loom::gen_canonical(w[0]);
+        f1[t]                     = loom::gen_canonical(w[1]);
+    }
 }

@@ -138,3 +138,31 @@ def ulp_mismatches(a, b, ulp=100):
     a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
     assert a.dtype == b.dtype and a.dtype.kind == "f" and len(a) == len(b)
     return L.ref_ulp_mismatches(a.ctypes.data, b.ctypes.data, len(a), ulp, a.dtype.itemsize)
+
+
+# the reference's random-number IR macros (oracle/ref_kernels.hpp rng_macros)
+RNG = "ref_rng_macros_f32"
+# counter words that overflow when bumped by 39916801, and their neighbours
+RNG_EDGES = [(0, 0), (0xFFFFFFFF, 0), (0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF - 39916800, 7),
+             (0xFFFFFFFF - 39916801, 7), (12345, 0xFFFFFFFF)]
+
+
+def rng_inputs(n, seed=0):
+    rng = np.random.default_rng([seed, 7])
+    c0 = rng.integers(0, 2**32, n, dtype=np.uint32)
+    c1 = rng.integers(0, 2**32, n, dtype=np.uint32)
+    for i, (a, b) in enumerate(RNG_EDGES[:n]):
+        c0[i], c1[i] = a, b
+    return [c0, c1]
+
+
+def rng_reference(c0, c1):
+    """(w0, w1, n0, n1, f0, f1) from the reference's C++ functions."""
+    L = O.lib(O.REF_PATH)
+    L.ref_rng.argtypes = [C.c_void_p] * 8 + [C.c_uint64]
+    L.ref_rng.restype = None
+    c0, c1 = np.ascontiguousarray(c0, np.uint32), np.ascontiguousarray(c1, np.uint32)
+    n = len(c0)
+    outs = [np.zeros(n, np.uint32) for _ in range(4)] + [np.zeros(n, np.float32) for _ in range(2)]
+    L.ref_rng(c0.ctypes.data, c1.ctypes.data, *[o.ctypes.data for o in outs], n)
+    return outs

@@ -49,3 +49,22 @@ def test_reference_program_parity(gpu, name):
         (got,) = r.outputs
         assert got.dtype == want.dtype
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (name, fname)
+
+
+def test_reference_rng_macros(gpu):
+    """The reference's random-number IR macros on the GPU, every form,
+    bit-exact against the oracle and the reference's C++ functions."""
+    load_ref_kernels()
+    cols = P.rng_inputs(N, seed=1)
+    want, err = O.run(P.RNG, cols, threads=16, path=O.REF_PATH)
+    assert err == 0
+    ref = P.rng_reference(*cols)
+    for w, r in zip(want, ref):
+        assert np.array_equal(w.view(np.uint32), r.view(np.uint32))
+    k = loom_amd.kernel(P.RNG)
+    for fname, mode, params in FORMS:
+        print(f"[gpu] {P.RNG} {fname} n={N}", flush=True)
+        r = loom_amd.run(k, mode, cols, params=loom_amd.make_params(mode=mode, **params))
+        assert r.errors == 0
+        for g, w in zip(r.outputs, want):
+            assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), fname

@@ -25,6 +25,7 @@ def test_every_program_registered():
     load_ref_kernels()
     names = {loom_amd.kernel(n).name for n in P.NAMES}
     assert len(names) == len(P.NAMES) == 80
+    assert loom_amd.kernel(P.RNG).outputs == [np.uint32] * 4 + [np.float32] * 2
     for name in P.NAMES:
         k = loom_amd.kernel(name)
         _, ins, outs = O.io(name, O.REF_PATH)
@@ -72,3 +73,33 @@ def test_wrong_validator_name_is_rejected():
     x = np.zeros(4, np.uint32)
     ip = (C.c_void_p * 16)(x.ctypes.data)
     assert L.ref_validate(b"ref_no_such_program", ip, x.ctypes.data, 4) == -1
+
+
+def test_rng_macros_match_reference_functions():
+    """The reference's Threefry / bump_ctr / gen_canonical IR macros through
+    the oracle equal its C++ functions bit for bit (the check of
+    src/tests/vm_smallpt.cpp:358-376), including counters that overflow."""
+    cols = P.rng_inputs(N)
+    got, err = O.run(P.RNG, cols, path=O.REF_PATH)
+    assert err == 0
+    want = P.rng_reference(*cols)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+
+
+def test_threefry_kernel_matches_reference_function():
+    """This repository's own Threefry-2x32 kernel (threefry2x32_u32) against
+    the reference's C++ Threefry-2x32 with the same key: the block words
+    agree bit for bit.  Its counter output is a 64-bit add (c1:c0 +
+    39916801), which equals the reference's bump_ctr except where the low
+    word wraps (bump_ctr then adds the wrapped low word to the high word and
+    zeroes the low word, src/common/random.hpp:93-103)."""
+    c0, c1 = P.rng_inputs(N)
+    (x0, x1, n0, n1), err = O.run("threefry2x32_u32", [c0, c1])
+    assert err == 0
+    w0, w1, b0, b1, _, _ = P.rng_reference(c0, c1)
+    assert np.array_equal(x0, w0) and np.array_equal(x1, w1)
+    wraps = c0.astype(np.uint64) + 39916801 > 0xFFFFFFFF
+    assert wraps.any() and (~wraps).any()
+    assert np.array_equal(n0[~wraps], b0[~wraps]) and np.array_equal(n1[~wraps], b1[~wraps])
+    assert np.array_equal(b0[wraps], np.zeros(wraps.sum(), np.uint32))
