@@ -38,7 +38,11 @@ def _run(cmd):
         raise RuntimeError(f"{' '.join(cmd[:3])} ... failed\n{r.stderr[-6000:]}")
 
 
-def build(ref: Path) -> list[Path]:
+def build(ref: Path, before_link=None) -> list[Path]:
+    """Build both libraries; `before_link` (if given) is called after the
+    objects are compiled and before the kernel library is linked against
+    libloom_amd.so, so a caller can compile this concurrently with that
+    library and make the link wait for it."""
     sys.path.insert(0, str(ROOT))
     from loom_amd import build as B
     src = ref / "src"
@@ -68,6 +72,8 @@ def build(ref: Path) -> list[Path]:
         fo = ex.submit(oracle)
         objs = list(ex.map(obj, SOURCES))
         fo.result()
+    if before_link is not None:
+        before_link()
     tmp = KERNEL_LIB.with_suffix(".tmp")
     _run([B.HIPCC, f"--offload-arch={B.ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
           f"-L{B.BUILD}", "-lloom_amd", f"-Wl,-rpath,{B.BUILD}",
