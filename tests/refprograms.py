@@ -166,3 +166,19 @@ def rng_reference(c0, c1):
     outs = [np.zeros(n, np.uint32) for _ in range(4)] + [np.zeros(n, np.float32) for _ in range(2)]
     L.ref_rng(c0.ctypes.data, c1.ctypes.data, *[o.ctypes.data for o in outs], n)
     return outs
+
+
+JMP4 = [n for n in NAMES if n.startswith("ref_jmp4_test_")]
+
+
+def jmp4_wide_inputs(name, n=4096, seed=0):
+    """Jump indexes over the whole range of the index type (0-255 first):
+    indexes past the four labels end the thread (the op contract), leaving
+    the index as the output, which is what the reference's validator returns
+    for them (src/tests/programs.hpp:325-337)."""
+    _, itys, _ = O.io(name, O.REF_PATH)
+    dt = np.dtype(itys[0])
+    rng = np.random.default_rng([seed, 4])
+    c = rng.integers(0, np.iinfo(dt).max, n, dtype=dt, endpoint=True)
+    c[:256] = np.arange(256, dtype=dt)
+    return [c]

@@ -68,3 +68,21 @@ def test_reference_rng_macros(gpu):
         assert r.errors == 0
         for g, w in zip(r.outputs, want):
             assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), fname
+
+
+@pytest.mark.parametrize("name", P.JMP4)
+def test_reference_jmp4_past_the_table(gpu, name):
+    """Jump indexes over the whole index range on the GPU (indexes past the
+    table halt), every form, against the oracle and the reference's
+    validator."""
+    load_ref_kernels()
+    cols = P.jmp4_wide_inputs(name, N, seed=1)
+    (want,), err = O.run(name, cols, threads=16, path=O.REF_PATH)
+    assert err == 0
+    assert np.array_equal(want, P.validate(name, cols))
+    k = loom_amd.kernel(name)
+    for fname, mode, params in FORMS:
+        print(f"[gpu] {name} wide {fname} n={N}", flush=True)
+        r = loom_amd.run(k, mode, cols, params=loom_amd.make_params(mode=mode, **params))
+        assert r.errors == 0
+        assert np.array_equal(r.outputs[0], want), fname

@@ -103,3 +103,13 @@ def test_threefry_kernel_matches_reference_function():
     assert wraps.any() and (~wraps).any()
     assert np.array_equal(n0[~wraps], b0[~wraps]) and np.array_equal(n1[~wraps], b1[~wraps])
     assert np.array_equal(b0[wraps], np.zeros(wraps.sum(), np.uint32))
+
+
+@pytest.mark.parametrize("name", P.JMP4)
+def test_jmp4_past_the_table_matches_reference_validator(name):
+    """The reference's validator returns the index itself for indexes past
+    the table; the op contract (such a jump halts) gives the same outputs."""
+    cols = P.jmp4_wide_inputs(name)
+    (got,), err = O.run(name, cols, path=O.REF_PATH)
+    assert err == 0
+    assert np.array_equal(got, P.validate(name, cols))
