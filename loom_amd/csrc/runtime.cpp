@@ -268,29 +268,48 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
         }
         return g_dev.pass_ev[ev_used++];
     };
-    for(;;)
+    if(!c.regroup)
     {
-        for(std::uint32_t k = 0; k < check_every; ++k)
-        {
-            hipEvent_t a = next_ev(), b = next_ev();
-            if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
-            LOOM_HIP(hipEventRecord(a, g_dev.stream));
-            LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
-            LOOM_HIP(hipEventRecord(b, g_dev.stream));
-            evs.emplace_back(a, b);
-            hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream, ws.ctl,
-                               nb, qcap, c.n, 0u);
-            LOOM_HIP(hipGetLastError());
-            ++passes;
-        }
-        LOOM_HIP(hipMemcpyAsync(g_dev.host_ctl, ws.ctl, sizeof(HybridCtl), hipMemcpyDeviceToHost,
-                                g_dev.stream));
-        LOOM_HIP(hipStreamSynchronize(g_dev.stream));
-        if(g_dev.host_ctl->done) break;
-        if(passes >= max_passes) return fail(LOOM_E_RUNTIME, "hybrid run exceeded %u passes", max_passes);
-        check_every = std::min<std::uint32_t>(check_every * 2, 8);
+        // One pass does all the work: every wave takes input until none is
+        // left and finishes its remaining lanes in place (a wave over its
+        // step budget raises the error flag instead), so no control-block
+        // read-back and host round trip is needed to learn that it is done.
+        hipEvent_t a = next_ev(), b = next_ev();
+        if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
+        LOOM_HIP(hipEventRecord(a, g_dev.stream));
+        LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
+        LOOM_HIP(hipEventRecord(b, g_dev.stream));
+        evs.emplace_back(a, b);
+        passes = 1;
+        LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
     }
-    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+    else
+        for(;;)
+        {
+            for(std::uint32_t k = 0; k < check_every; ++k)
+            {
+                hipEvent_t a = next_ev(), b = next_ev();
+                if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
+                LOOM_HIP(hipEventRecord(a, g_dev.stream));
+                LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
+                LOOM_HIP(hipEventRecord(b, g_dev.stream));
+                evs.emplace_back(a, b);
+                hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream,
+                                   ws.ctl, nb, qcap, c.n, 0u);
+                LOOM_HIP(hipGetLastError());
+                ++passes;
+            }
+            LOOM_HIP(hipMemcpyAsync(g_dev.host_ctl, ws.ctl, sizeof(HybridCtl),
+                                    hipMemcpyDeviceToHost, g_dev.stream));
+            // the end of the run, if this check finds it done: recorded
+            // before the host waits, so the host's round trip is not timed
+            LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+            LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+            if(g_dev.host_ctl->done) break;
+            if(passes >= max_passes)
+                return fail(LOOM_E_RUNTIME, "hybrid run exceeded %u passes", max_passes);
+            check_every = std::min<std::uint32_t>(check_every * 2, 8);
+        }
     LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
     float total = 0;
     LOOM_HIP(hipEventElapsedTime(&total, g_dev.ev[0], g_dev.ev[1]));
