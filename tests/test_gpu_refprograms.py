@@ -86,3 +86,22 @@ def test_reference_jmp4_past_the_table(gpu, name):
         r = loom_amd.run(k, mode, cols, params=loom_amd.make_params(mode=mode, **params))
         assert r.errors == 0
         assert np.array_equal(r.outputs[0], want), fname
+
+
+def test_reference_fib_at_full_size(gpu):
+    """The fib workload at its full size (2^20 threads, n in [0, 90]) through
+    the reference's own fib_test program, compact and hybrid, against the
+    reference's own validator val_fib; this repository's fib_u64 kernel
+    gives the same outputs on the same inputs."""
+    from loom_amd import workloads
+    load_ref_kernels()
+    n = workloads.fib_inputs()
+    want = P.validate("ref_fib_test_u64", [n])
+    k = loom_amd.kernel("ref_fib_test_u64")
+    for fname, mode, params in FORMS:
+        print(f"[gpu] ref_fib_test_u64 2^20 {fname}", flush=True)
+        r = loom_amd.run(k, mode, [n], params=loom_amd.make_params(mode=mode, **params))
+        assert r.errors == 0
+        assert np.array_equal(r.outputs[0], want), fname
+    own = loom_amd.run("fib_u64", abi.MODE_HYBRID, [n])
+    assert np.array_equal(own.outputs[0], want)

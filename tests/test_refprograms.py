@@ -113,3 +113,16 @@ def test_jmp4_past_the_table_matches_reference_validator(name):
     (got,), err = O.run(name, cols, path=O.REF_PATH)
     assert err == 0
     assert np.array_equal(got, P.validate(name, cols))
+
+
+def test_reference_fib_at_full_size():
+    """The fib workload at its full size (2^20 threads, n in [0, 90]) through
+    the oracle running the reference's fib_test equals the reference's
+    val_fib, and this repository's fib_u64 kernel agrees with both."""
+    from loom_amd import workloads
+    n = workloads.fib_inputs()
+    want = P.validate("ref_fib_test_u64", [n])
+    (got,), err = O.run("ref_fib_test_u64", [n], path=O.REF_PATH)
+    assert err == 0 and np.array_equal(got, want)
+    (own,), err = O.run("fib_u64", [n])
+    assert err == 0 and np.array_equal(own, want)
