@@ -7,7 +7,9 @@ front-end, from the reference checkout where it lies, into oracle/_ref/:
   (ref_validators.cpp);
 * libloom_refkernels.so — the same kernels lowered for gfx950 by the
   product's backend; loading it (after the product library) registers them
-  with the product library, where the C ABI finds them by name.
+  with the product library, where the C ABI finds them by name;
+* ref_facade_test — a host program that drives the reference's test
+  programs through the C++ facade (ref_facade_test.cpp).
 
 Only tests load these.  Nothing is built when the reference is absent (the
 GPU box has none; the libraries built here travel with the tree).
@@ -28,6 +30,7 @@ ROOT = HERE.parent
 OUT = HERE / "_ref"
 ORACLE_LIB = OUT / "liboracle_ref.so"
 KERNEL_LIB = OUT / "libloom_refkernels.so"
+FACADE_BIN = OUT / "ref_facade_test"
 SOURCES = ["ref_pt_const.hip", "ref_pt2_const.hip", "ref_pt2_const_1024.hip",
            "ref_programs_a.hip", "ref_programs_b.hip", "ref_programs_c.hip"]
 
@@ -81,7 +84,17 @@ def build(ref: Path, before_link=None) -> list[Path]:
     os.replace(tmp, KERNEL_LIB)
     for o in objs:
         o.unlink()
-    return [ORACLE_LIB, KERNEL_LIB]
+    # linked with --no-as-needed: nothing references the kernel library by
+    # symbol, its load registers the kernels the facade looks up
+    tmp = FACADE_BIN.with_suffix(".tmp")
+    _run([os.environ.get("CXX", "g++"), "-std=c++20", "-O2", "-ffp-contract=off",
+          f"-I{ROOT / 'include' / 'loom_vm'}", f"-I{ROOT / 'include'}", f"-I{src}", f"-I{ext}",
+          str(HERE / "ref_facade_test.cpp"), "-o", str(tmp), f"-L{B.BUILD}", f"-L{OUT}",
+          "-Wl,--no-as-needed", "-lloom_refkernels", "-lloom_amd", "-Wl,--as-needed",
+          f"-Wl,-rpath,{OUT}", f"-Wl,-rpath,{B.BUILD}", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath,$ORIGIN/../../loom_amd/_build"])
+    os.replace(tmp, FACADE_BIN)
+    return [ORACLE_LIB, KERNEL_LIB, FACADE_BIN]
 
 
 def main(argv=None) -> int:

@@ -105,3 +105,20 @@ def test_reference_fib_at_full_size(gpu):
         assert np.array_equal(r.outputs[0], want), fname
     own = loom_amd.run("fib_u64", abi.MODE_HYBRID, [n])
     assert np.array_equal(own.outputs[0], want)
+
+
+def test_reference_programs_through_the_cpp_facade(gpu):
+    """oracle/_ref/ref_facade_test: the reference's own test programs driven
+    through the C++ facade with the reference's call shapes (a function per
+    program, the stream protocol for factorial, as its CUDA backend test
+    does), checked against its validators; exits non-zero on a mismatch."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "ref_facade_test"
+    if not exe.exists():
+        pytest.skip("reference facade test not built (no reference checkout)")
+    print("[gpu] running", exe, flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-4000:], flush=True)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "ref_facade_test: ok" in r.stdout
