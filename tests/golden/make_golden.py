@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py work     # work_totals.json only
+    python tests/golden/make_golden.py frame    # smallpt9_1024x768x64.json only
 
 The reference holds no golden vectors for this path and cannot be built
 here, so the fixtures come from computations independent of the oracle's
@@ -15,6 +16,9 @@ block interpreter and of the GPU backend:
 * smallpt9_32x24x8.npz — a 32x24 frame at 8 spp of the 9-sphere path
   tracer from the direct C++ restatement (not the IR): per-sample radiance
   bits and the per-pixel image.
+* smallpt9_1024x768x64.json — the whole path-tracing workload frame
+  (50,331,648 sample threads) from the direct C++ restatement: SHA-256 of
+  each radiance channel's bits and a sample of (sample index, bits) rows.
 * literals.json — literal strings with the correctly rounded float32 and
   float64 values (Python's parser) beside the values of the reference's
   literal parser as restated by the oracle.
@@ -162,6 +166,21 @@ def smallpt_fixture():
                         image=img)
 
 
+def frame_fixture():
+    """The 1024x768x64 frame by the direct C++ path tracer (not the IR)."""
+    from tests import oracle_lib as O
+    w, h, spp = 1024, 768, 64
+    sid, job = workloads.pt_inputs(w, h, spp)
+    cols = O.smallpt9_direct(sid, job, threads=8)
+    rows = sample_rows(len(sid), 303)
+    return {"kernel": "smallpt9_f32", "w": w, "h": h, "spp": spp, "threads": len(sid),
+            "job": int(job[0]), "inputs_sha256": sha(sid),
+            "outputs_sha256": [sha(c.view(np.uint32)) for c in cols],
+            "sample": [[int(i)] + [int(c.view(np.uint32)[i]) for c in cols] for i in rows],
+            "how": "direct C++ restatement of the path tracer (oracle smallpt9_direct), "
+                   "radiance bits per channel"}
+
+
 LITERALS = ["0", "1", "0.5", "0.1", "0.25", "2.5", "1e-12", "1e-6", "1e5", "1e30", "-0.1",
             "6.283185307179586", "3.14159265", "0.093073", "0.999", "81.6", "681.33",
             "0.042612", "0.5135", "295.6", "1.5", "0.75", "12", "16777216", "16777217",
@@ -185,6 +204,10 @@ def literal_fixture():
 
 
 def main():
+    if sys.argv[1:] == ["frame"]:
+        (HERE / "smallpt9_1024x768x64.json").write_text(json.dumps(frame_fixture(), indent=0) + "\n")
+        print("frame", flush=True)
+        return
     if sys.argv[1:] == ["work"]:
         (HERE / "work_totals.json").write_text(json.dumps(work_fixture(), indent=1) + "\n")
         print("work totals", flush=True)
@@ -195,6 +218,8 @@ def main():
     print("collatz", flush=True)
     smallpt_fixture()
     print("smallpt9", flush=True)
+    (HERE / "smallpt9_1024x768x64.json").write_text(json.dumps(frame_fixture(), indent=0) + "\n")
+    print("frame", flush=True)
     (HERE / "literals.json").write_text(json.dumps(literal_fixture(), indent=0) + "\n")
     print("literals", flush=True)
     (HERE / "work_totals.json").write_text(json.dumps(work_fixture(), indent=1) + "\n")

@@ -121,3 +121,18 @@ def test_work_totals_path_tracer_prefix():
     assert got == (f["steps"], f["ops"], f["flops"])
     # every sample runs START, then at least one intersection
     assert f["steps"] >= 2 * 65536 and f["flops"] > 0
+
+
+def test_full_frame_fixture_against_the_ir():
+    """The whole-frame fixture (direct C++ restatement) at its sampled rows
+    equals the oracle's block interpreter running the kernel's IR."""
+    f = json.loads((G / "smallpt9_1024x768x64.json").read_text())
+    sid, job = workloads.pt_inputs(f["w"], f["h"], f["spp"])
+    assert hashlib.sha256(sid.tobytes()).hexdigest() == f["inputs_sha256"]
+    assert int(job[0]) == f["job"] and len(sid) == f["threads"]
+    rows = np.array([r[0] for r in f["sample"]], np.int64)
+    got, err = O.run("smallpt9_f32", [sid[rows], job[rows]])
+    assert err == 0
+    for c in range(3):
+        want = np.array([r[1 + c] for r in f["sample"]], np.uint32)
+        assert np.array_equal(got[c].view(np.uint32), want)

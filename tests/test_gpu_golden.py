@@ -55,3 +55,24 @@ def test_smallpt9_frame(gpu, mode, params):
     out = run("smallpt9_f32", ins, mode, params)
     for got, key in zip(out, ("lx", "ly", "lz")):
         assert np.array_equal(got.view(np.uint32), z[key])
+
+
+FRAME_FORMS = [("compact_sm", abi.MODE_COMPACT, {}),
+               ("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+               ("hybrid_tuned", abi.MODE_HYBRID, None)]
+
+
+@pytest.mark.parametrize("fname,mode,params", FRAME_FORMS)
+def test_smallpt9_full_frame(gpu, fname, mode, params):
+    """The whole path-tracing workload frame (1024x768 at 64 spp, 50,331,648
+    sample threads) bit for bit against the fixture from the direct C++
+    restatement, per radiance channel."""
+    from loom_amd import autotune
+    f = json.loads((G / "smallpt9_1024x768x64.json").read_text())
+    ins = workloads.pt_inputs(f["w"], f["h"], f["spp"])
+    if params is None:
+        params = autotune.best_params("smallpt9_f32")
+    out = run("smallpt9_f32", ins, mode, params)
+    print(f"[gpu] full frame {fname}: comparing", flush=True)
+    for c, got in enumerate(out):
+        assert sha(got.view(np.uint32)) == f["outputs_sha256"][c], (fname, c)
