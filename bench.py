@@ -294,6 +294,16 @@ def main():
                    "backend": "hybrid", "hybrid_params": hyb},
         "active_lane_fraction": h_last.active_lane_fraction,
     }
+    # the measured frame against the committed whole-frame fixture (SHA-256
+    # of each radiance channel, from the direct CPU path tracer)
+    fixture = ROOT / "tests" / "golden" / "smallpt9_1024x768x64.json"
+    if fixture.exists():
+        import hashlib
+        f = json.loads(fixture.read_text())
+        if (f["w"], f["h"], f["spp"]) == (W, H, SPP):
+            result["outputs_match_fixture"] = all(
+                hashlib.sha256(np.ascontiguousarray(x.view(np.uint32)).tobytes()).hexdigest() == d
+                for x, d in zip(h_img, f["outputs_sha256"]))
 
     if not a.no_compact:
         steps(abi.MODE_COMPACT, cp, 1)
