@@ -212,6 +212,8 @@ def integer_configs(steps: int):
     import loom_amd
     from loom_amd import abi, autotune, workloads
     out = {}
+    import hashlib
+    fixtures = {"fib_u64": "fib_u64_c1.json", "collatz_u64": "collatz_u64_c2.json"}
     for key, name, cols in (("fib_u64_2^20", "fib_u64", [workloads.fib_inputs(1 << 20)]),
                             ("collatz_u64_2^24", "collatz_u64",
                              [workloads.collatz_inputs(1 << 24)])):
@@ -239,6 +241,14 @@ def integer_configs(steps: int):
                            "active_lane_fraction": r.active_lane_fraction,
                            "outputs_identical": all(np.array_equal(x, y)
                                                     for x, y in zip(got, first))}
+        # the whole workload's outputs against the committed fixture
+        # (computed from the recurrence / brute force, not the oracle)
+        fx = ROOT / "tests" / "golden" / fixtures[name]
+        if fx.exists():
+            f = json.loads(fx.read_text())
+            if f["threads"] == n:
+                entry["outputs_match_fixture"] = hashlib.sha256(
+                    np.ascontiguousarray(first[0]).tobytes()).hexdigest() == f["outputs_sha256"]
         out[key] = entry
     return out
 
