@@ -233,6 +233,86 @@ int loom_run_host(int kernel, int mode, const void* const* inputs,
                   void* const* outputs, uint64_t n, const loom_params* params,
                   loom_run_stats* stats);
 
+/* ---- execution contexts ------------------------------------------------------
+ * Several host threads feeding one GPU: the reference runs many worker
+ * threads, each stepping its own streams of every kernel, while the main
+ * thread exchanges inputs and outputs (src/vm/mp.hpp:61-131 Context, 158-190
+ * Worker, 316-385 Shop).  A context is a host thread's own lane to the
+ * device: a kernel stream and a copy stream (default priority), its error
+ * flags, counters and workspaces.  Calls on different contexts run
+ * concurrently and their kernels overlap on the device; calls on one
+ * context serialise on its lock.  The entry points without a context
+ * argument use the default context made by loom_init, and are safe to call
+ * from several threads (they serialise). */
+typedef struct loom_context loom_context;
+int loom_context_create(loom_context** ctx);
+int loom_context_destroy(loom_context* ctx);
+/* loom_run, loom_run_host, loom_batch_run, loom_fill, loom_copy_*,
+ * loom_synchronize on context ctx (NULL: the default context). */
+int loom_ctx_run(loom_context* ctx, int kernel, int mode, const void* const* inputs,
+                 void* const* outputs, uint64_t n, const loom_params* params,
+                 loom_run_stats* stats);
+int loom_ctx_run_host(loom_context* ctx, int kernel, int mode, const void* const* inputs,
+                      void* const* outputs, uint64_t n, const loom_params* params,
+                      loom_run_stats* stats);
+int loom_ctx_fill(loom_context* ctx, void* dst, const void* pattern, size_t elem_bytes,
+                  uint64_t count);
+struct loom_job;
+int loom_ctx_batch_run(loom_context* ctx, int set, const struct loom_job* jobs, int njobs,
+                       const loom_params* params, loom_run_stats* stats);
+int loom_ctx_copy_async(loom_context* ctx, void* dst, const void* src, size_t bytes, int kind,
+                        uint64_t* ticket);
+int loom_ctx_copy_status(loom_context* ctx, uint64_t ticket, int* done);
+int loom_ctx_copy_wait(loom_context* ctx, uint64_t ticket);
+int loom_ctx_synchronize(loom_context* ctx);
+
+/* ---- device-resident stream ----------------------------------------------------
+ * Replaces hybrid_stream<K> / the concurrent stream facade
+ * (src/vm/simd.hpp:4319-4327, 2797-2903; bve::MemoryFacade and
+ * exchange_from, src/vm/bve.hpp:1028-1229).  `slots` thread slots whose
+ * registers stay in device memory (structure of arrays, one column per
+ * register) between steps:
+ *   set_inputs  starts threads 0..m-1 (m = min(n, slots)) in the slots and
+ *               empties the rest;
+ *   step        one launch: every live slot runs under loom_params.reconv
+ *               until at most live_threshold lanes of its wave are live
+ *               (0: to HALT) or max_steps dispatches were made; threads keep
+ *               their registers between steps;
+ *   sort        reports whether an exchange is available: at least 64 slots
+ *               halted since the last exchange, or some halted and none live;
+ *   exchange    for m = min(n, halted) finished threads, in the order they
+ *               halted: their outputs (and sequence numbers, if ids is not
+ *               NULL) go to the host columns, and the next m inputs start
+ *               in their slots; only those m rows cross PCIe;
+ *   drain       the outputs of up to cap finished threads, whose slots become
+ *               empty (the reference's get_output_zipped at the end);
+ *   state       live threads (done: live == 0) and finished threads waiting.
+ * Threads are numbered in the order they were given: set_inputs numbers
+ * them 0.., every exchange continues the numbering.  The reference returns
+ * a stream's outputs unordered; the sequence numbers let a caller restore
+ * input order.  Each stream owns a HIP stream; calls on one stream
+ * serialise on its lock. */
+typedef struct loom_stream loom_stream;
+typedef struct loom_stream_counters
+{
+    uint64_t steps, exchanges;
+    uint64_t rows_in, rows_out;    /* threads started / outputs returned */
+    uint64_t bytes_h2d, bytes_d2h; /* bytes copied in / out by this stream */
+} loom_stream_counters;
+int loom_stream_create(int kernel, uint64_t slots, const loom_params* params,
+                       loom_stream** stream);
+int loom_stream_destroy(loom_stream* stream);
+int loom_stream_set_inputs(loom_stream* stream, const void* const* inputs, uint64_t n,
+                           uint64_t* taken);
+int loom_stream_step(loom_stream* stream, loom_run_stats* stats);
+int loom_stream_sort(loom_stream* stream, int* exchange_available);
+int loom_stream_exchange(loom_stream* stream, const void* const* inputs, uint64_t n,
+                         void* const* outputs, uint64_t* ids, uint64_t* exchanged);
+int loom_stream_drain(loom_stream* stream, void* const* outputs, uint64_t* ids, uint64_t cap,
+                      uint64_t* drained);
+int loom_stream_state(loom_stream* stream, uint64_t* live, uint64_t* halted);
+int loom_stream_counters_get(loom_stream* stream, loom_stream_counters* counters);
+
 /* ---- batched launches ------------------------------------------------------
  * Replaces the multi-kernel Shop (src/vm/mp.hpp:316-385: bind_inputs, the
  * workers' round-robin over per-kernel streams, finalize): the jobs of one
@@ -300,6 +380,8 @@ typedef struct loom_request
     double      done_ms;               /* completion (outputs on the host) */
     uint32_t    steps;                 /* steps the request took part in */
     int32_t     status;                /* 0, or the error code of the request */
+    int32_t     complete;              /* set (release) when done_ms and status are final */
+    int32_t     reserved;
 } loom_request;
 
 typedef struct loom_server_stats
@@ -319,6 +401,19 @@ int loom_server_create(const loom_server_config* config, loom_server** server);
  * realtime == 0 treats every request as arrived at the start. */
 int loom_server_run(loom_server* server, loom_request* requests, uint64_t count,
                     int realtime, loom_server_stats* stats);
+/* Online serving: the loop runs on a host thread of its own and requests
+ * are submitted from any host thread while it runs (the reference Shop's
+ * main thread exchanging while the workers compute, src/vm/mp.hpp:349-369).
+ * A submitted request's arrival is the time of its submission; the caller
+ * keeps the request and its host columns alive until it is complete
+ * (loom_server_poll / loom_server_wait; wait returns its status).
+ * loom_server_stop stops accepting, completes every submitted request and
+ * joins the loop.  loom_server_run and the online loop are exclusive. */
+int loom_server_start(loom_server* server);
+int loom_server_submit(loom_server* server, loom_request* request);
+int loom_server_poll(loom_server* server, const loom_request* request, int* done);
+int loom_server_wait(loom_server* server, const loom_request* request);
+int loom_server_stop(loom_server* server, loom_server_stats* stats);
 int loom_server_destroy(loom_server* server);
 
 #ifdef __cplusplus

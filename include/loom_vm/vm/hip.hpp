@@ -21,12 +21,18 @@
 #include <vm/lower.hpp>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <iterator>
+#include <memory>
+#include <mutex>
 #include <ranges>
+#include <thread>
 #include <tuple>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace loom::vm::hip
@@ -345,11 +351,18 @@ template <Kernel K>
 using hybrid_function = function<K, LOOM_MODE_HYBRID>;
 
 /// @brief Streaming invocation (reference hybrid_stream with the
-/// ConcurrentStreamFacade protocol, simd.hpp:2797-2903, bve.hpp:1208-1229):
-/// a pool of `slots` resident threads; `set_inputs` fills it, `s()` runs the
-/// admitted threads on the GPU, `sort()` reports whether halted slots can be
-/// exchanged, `exchange(rest)` swaps the outputs of halted slots for new
-/// inputs (one for one), `done()` is true when no admitted thread is left.
+/// concurrent stream facade protocol, simd.hpp:2797-2903,
+/// bve.hpp:1208-1229), over the device-resident stream of the C ABI
+/// (loom_stream_*): `slots` thread slots whose registers stay on the GPU;
+/// `set_inputs` fills them, `s()` advances the live slots (each wave until
+/// at most `live_threshold` of its lanes are live), `sort()` reports
+/// whether halted slots can be exchanged, `exchange(rest)` swaps the outputs
+/// of halted threads for new inputs (one for one, only those rows cross
+/// PCIe), `done()` is true when no thread is live, `get_output_zipped()`
+/// returns the outputs still held.  Outputs come in the order threads
+/// halted, as the reference's come in its sorted order.  Calls may come from
+/// two threads at once (a worker stepping, the main thread exchanging, as
+/// mp::Context does, mp.hpp:67-131): the C ABI serialises them.
 template <Kernel K>
 class hybrid_stream
 {
@@ -357,112 +370,117 @@ class hybrid_stream
     using out_t = output_tuple<K>;
     using row_t = output_type<K>;
 
-    std::size_t       m_slots;
-    params            m_p;
-    Columns<in_t>     m_in;     // slot inputs
-    Columns<out_t>    m_out;    // slot outputs
-    std::vector<char> m_state;  // 0 empty, 1 admitted, 2 halted (holds outputs)
+    loom_stream*               m_s = nullptr;
+    std::size_t                m_slots;
+    Columns<in_t>              m_in;  // staging of rows going in
+    Columns<out_t>             m_out; // staging of rows coming out
+    std::vector<std::uint64_t> m_ids;
+    loom_run_stats             m_stats{};
+
+    template <typename It, typename End>
+    std::size_t
+    stage(It& it, const End& end, std::size_t cap) noexcept
+    {
+        std::size_t m = 0;
+        for(; m < cap && it != end; ++m, ++it) m_in.set_row_value(m, *it);
+        return m;
+    }
+
+    std::vector<row_t>
+    rows(std::size_t m) const noexcept
+    {
+        std::vector<row_t> res(m);
+        for(std::size_t i = 0; i < m; ++i) res[i] = m_out.row_value(i);
+        return res;
+    }
 
   public:
     using program = K;
 
-    explicit hybrid_stream(std::size_t slots, params p = {}) noexcept : m_slots(slots), m_p(p)
+    explicit hybrid_stream(std::size_t slots, params p = {}) noexcept :
+        m_slots((std::max<std::size_t>(slots, 1) + 63) / 64 * 64)
     {
-        m_in.resize(slots);
-        m_out.resize(slots);
-        m_state.assign(slots, 0);
+        const loom_params c = detail::to_c(p, LOOM_MODE_HYBRID);
+        detail::check(loom_stream_create(kernel_index<K>(), m_slots, &c, &m_s),
+                      "loom_stream_create");
+        m_in.resize(m_slots);
+        m_out.resize(m_slots);
+        m_ids.resize(m_slots);
     }
 
-    /// @brief Fill empty slots from @p input (rows of the input type);
-    /// returns the iterator past the consumed rows.
+    hybrid_stream(const hybrid_stream&)            = delete;
+    hybrid_stream& operator=(const hybrid_stream&) = delete;
+
+    ~hybrid_stream() { loom_stream_destroy(m_s); }
+
+    /// @brief Start the first rows of @p input in the slots (the stream
+    /// starts over); returns the iterator past the rows taken.
     template <std::ranges::input_range R>
     auto
     set_inputs(R&& input) noexcept
     {
-        auto it = std::ranges::begin(input);
-        for(std::size_t s = 0; s < m_slots && it != std::ranges::end(input); ++s)
-            if(m_state[s] == 0)
-            {
-                m_in.set_row_value(s, *it);
-                m_state[s] = 1;
-                ++it;
-            }
+        auto              it = std::ranges::begin(input);
+        const std::size_t m  = stage(it, std::ranges::end(input), m_slots);
+        auto              ip = m_in.cptrs();
+        std::uint64_t     taken = 0;
+        detail::check(loom_stream_set_inputs(m_s, ip.data(), m, &taken), "loom_stream_set_inputs");
         return it;
     }
 
-    /// @brief Run every admitted thread to HALT (one GPU launch).
+    /// @brief One step of every live thread on the GPU.
     void
     operator()() noexcept
     {
-        std::vector<std::size_t> idx;
-        for(std::size_t s = 0; s < m_slots; ++s)
-            if(m_state[s] == 1) idx.push_back(s);
-        if(idx.empty()) return;
-        Columns<in_t>  in;
-        Columns<out_t> out;
-        in.resize(idx.size());
-        out.resize(idx.size());
-        for(std::size_t i = 0; i < idx.size(); ++i) in.set_row(i, m_in.row(idx[i]));
-        const loom_params c    = detail::to_c(m_p, LOOM_MODE_HYBRID);
-        auto              ip   = in.cptrs();
-        auto              op   = out.ptrs();
-        loom_run_stats    st{};
-        detail::check(loom_run_host(kernel_index<K>(), LOOM_MODE_HYBRID, ip.data(), op.data(),
-                                    idx.size(), &c, &st),
-                      "loom_run_host");
-        for(std::size_t i = 0; i < idx.size(); ++i)
-        {
-            [&]<std::size_t... I>(std::index_sequence<I...>)
-            { ((std::get<I>(m_out.cols)[idx[i]] = std::get<I>(out.cols)[i]), ...); }(
-                std::make_index_sequence<std::tuple_size_v<out_t>>{});
-            m_state[idx[i]] = 2;
-        }
+        detail::check(loom_stream_step(m_s, &m_stats), "loom_stream_step");
     }
 
-    /// @brief True when halted slots hold outputs to exchange.
+    /// @brief True when halted threads' outputs can be exchanged.
     bool
     sort() noexcept
     {
-        return std::ranges::count(m_state, char(2)) > 0;
+        int avail = 0;
+        detail::check(loom_stream_sort(m_s, &avail), "loom_stream_sort");
+        return avail != 0;
     }
 
-    /// @brief Outputs of halted slots, each slot refilled with the next row
-    /// of @p input; as many as both allow.
+    /// @brief Outputs of halted threads, their slots given the next rows of
+    /// @p input; as many as both allow.
     template <std::ranges::input_range R>
     std::vector<row_t>
     exchange(R&& input) noexcept
     {
-        std::vector<row_t> res;
-        auto               it = std::ranges::begin(input);
-        for(std::size_t s = 0; s < m_slots && it != std::ranges::end(input); ++s)
-            if(m_state[s] == 2)
-            {
-                res.push_back(m_out.row_value(s));
-                m_in.set_row_value(s, *it);
-                m_state[s] = 1;
-                ++it;
-            }
-        return res;
+        std::uint64_t live = 0, halted = 0;
+        detail::check(loom_stream_state(m_s, &live, &halted), "loom_stream_state");
+        auto              it = std::ranges::begin(input);
+        const std::size_t m  = stage(it, std::ranges::end(input), halted);
+        if(m == 0) return {};
+        auto          ip = m_in.cptrs();
+        auto          op = m_out.ptrs();
+        std::uint64_t got = 0;
+        detail::check(loom_stream_exchange(m_s, ip.data(), m, op.data(), m_ids.data(), &got),
+                      "loom_stream_exchange");
+        return rows(got);
     }
 
     bool
     done() const noexcept
     {
-        return std::ranges::count(m_state, char(1)) == 0;
+        std::uint64_t live = 0;
+        detail::check(loom_stream_state(m_s, &live, nullptr), "loom_stream_state");
+        return live == 0;
     }
 
-    /// @brief Outputs still held by halted slots (then the slots are empty).
+    /// @brief Outputs still held by halted threads (their slots become empty).
     std::vector<row_t>
     get_output_zipped() noexcept
     {
-        std::vector<row_t> res;
-        for(std::size_t s = 0; s < m_slots; ++s)
-            if(m_state[s] == 2)
-            {
-                res.push_back(m_out.row_value(s));
-                m_state[s] = 0;
-            }
-        return res;
+        std::uint64_t halted = 0;
+        detail::check(loom_stream_state(m_s, nullptr, &halted), "loom_stream_state");
+        auto          op  = m_out.ptrs();
+        std::uint64_t got = 0;
+        detail::check(loom_stream_drain(m_s, op.data(), m_ids.data(), halted, &got),
+                      "loom_stream_drain");
+        return rows(got);
     }
 
     std::size_t
@@ -470,5 +488,257 @@ class hybrid_stream
     {
         return m_slots;
     }
+
+    const loom_run_stats&
+    stats() const noexcept
+    {
+        return m_stats;
+    }
+
+    loom_stream_counters
+    counters() const noexcept
+    {
+        loom_stream_counters c{};
+        detail::check(loom_stream_counters_get(m_s, &c), "loom_stream_counters_get");
+        return c;
+    }
 };
+
+/// @brief Kernel execution context of a worker (reference mp::Context,
+/// mp.hpp:61-131): a stream of one kernel, a lock that orders sorting
+/// against exchanging, and the flag that an exchange is available.  A
+/// worker thread steps and sorts (`try_step`), the main thread exchanges
+/// (`try_exchange`); stepping may run while an exchange does.
+template <template <typename> class Stream, typename K>
+struct basic_context
+{
+    Stream<K>        stream;
+    std::mutex       lock;
+    bool             need_sort = false;
+    std::atomic_flag exchange_available;
+
+    template <typename... A>
+    explicit basic_context(A&&... a) noexcept : stream(std::forward<A>(a)...)
+    {
+    }
+
+    void
+    try_sort() noexcept
+    {
+        if(need_sort && lock.try_lock())
+        {
+            const bool ex = stream.sort();
+            lock.unlock();
+            need_sort = false;
+            if(ex) exchange_available.test_and_set(std::memory_order_relaxed);
+        }
+    }
+
+    void
+    try_step() noexcept
+    {
+        if(!(need_sort || stream.done()))
+        {
+            stream();
+            need_sort = true;
+        }
+        try_sort();
+    }
+
+    template <typename It, typename Sentinel, typename Out>
+    bool
+    try_exchange(It& ib, const Sentinel s, Out& out) noexcept
+    {
+        if(ib != s && exchange_available.test(std::memory_order_relaxed) && lock.try_lock())
+        {
+            for(const auto& ex: stream.exchange(std::ranges::subrange(ib, s)))
+            {
+                *out = ex;
+                ++out;
+                ++ib;
+            }
+            lock.unlock();
+            exchange_available.clear(std::memory_order_relaxed);
+            return true;
+        }
+        return false;
+    }
+};
+
+/// @brief Several host threads feeding one GPU (reference mp::Shop,
+/// mp.hpp:148-385, with a dynamic number of workers): every worker owns one
+/// context (a device-resident stream) per kernel and round-robins over them,
+/// stepping and sorting; the main thread binds the inputs, exchanges until
+/// every input is taken, signals the workers (`set()`), which finish their
+/// streams, and collects the last outputs (`finalize`).  Each stream has a
+/// HIP stream of its own, so the workers' launches overlap on the device.
+template <template <typename> class Stream, typename... Ks>
+class basic_shop
+{
+    using bench_t = std::tuple<std::unique_ptr<basic_context<Stream, Ks>>...>;
+    std::vector<bench_t> m_benches;
+    std::atomic<bool>    m_signal{false};
+
+    template <typename F>
+    void
+    visit(F&& f)
+    {
+        for(auto& b: m_benches) f(b);
+    }
+
+  public:
+    /// @brief Joins the worker threads when it goes out of scope (the
+    /// reference thread_pool).
+    class thread_pool
+    {
+        std::vector<std::thread> m_threads;
+
+      public:
+        explicit thread_pool(std::vector<std::thread>&& t) noexcept : m_threads(std::move(t)) {}
+        thread_pool(thread_pool&&) noexcept = default;
+        ~thread_pool()
+        {
+            for(auto& t: m_threads)
+                if(t.joinable()) t.join();
+        }
+        std::size_t
+        size() const noexcept
+        {
+            return m_threads.size();
+        }
+    };
+
+    template <typename... A>
+    basic_shop(std::size_t workers, A&&... stream_args) noexcept
+    {
+        m_benches.resize(std::max<std::size_t>(workers, 1));
+        for(auto& b: m_benches)
+            std::apply([&](auto&... ctx)
+                       { ((ctx = std::make_unique<typename std::remove_reference_t<decltype(ctx)>::element_type>(
+                               stream_args...)),
+                          ...); },
+                       b);
+    }
+
+    /// @brief Signal the workers that no input is left (reference Signal::set).
+    void
+    set() noexcept
+    {
+        m_signal.store(true, std::memory_order_release);
+    }
+
+    /// @brief Start one thread per bench; each steps its contexts until the
+    /// signal is set and every stream of its bench is done.
+    [[nodiscard]] thread_pool
+    get_thread_pool()
+    {
+        m_signal.store(false, std::memory_order_release);
+        std::vector<std::thread> t;
+        for(auto& b: m_benches)
+            t.emplace_back(
+                [this, &b]
+                {
+                    std::array<bool, sizeof...(Ks)> done{};
+                    while(!std::all_of(done.begin(), done.end(), [](bool d) { return d; }))
+                        [&]<std::size_t... I>(std::index_sequence<I...>)
+                        {
+                            (
+                                [&]
+                                {
+                                    auto& ctx = *std::get<I>(b);
+                                    if(done[I]) return;
+                                    if(m_signal.load(std::memory_order_acquire) && ctx.lock.try_lock())
+                                    {
+                                        if(ctx.stream.done()) done[I] = true;
+                                        ctx.lock.unlock();
+                                    }
+                                    ctx.try_step();
+                                }(),
+                                ...);
+                        }(std::index_sequence_for<Ks...>{});
+                });
+        return thread_pool(std::move(t));
+    }
+
+    /// @brief Give every stream its first inputs; returns, per kernel, the
+    /// pair (next input, end) of what is left.
+    template <typename... Inputs>
+        requires(sizeof...(Inputs) == sizeof...(Ks))
+    [[nodiscard]] auto
+    bind_inputs(Inputs&... inputs)
+    {
+        auto ret = std::make_tuple(std::make_pair(std::ranges::begin(inputs), std::ranges::end(inputs))...);
+        visit(
+            [&](auto& bench)
+            {
+                [&]<std::size_t... I>(std::index_sequence<I...>)
+                {
+                    ((std::get<I>(ret).first = std::get<I>(bench)->stream.set_inputs(
+                          std::ranges::subrange(std::get<I>(ret).first, std::get<I>(ret).second))),
+                     ...);
+                }(std::index_sequence_for<Ks...>{});
+            });
+        return ret;
+    }
+
+    /// @brief One exchange attempt on every context; true if any exchanged.
+    template <typename InputCollection, typename OutputCollection>
+    bool
+    exchange(InputCollection& ic, OutputCollection& oc) noexcept
+    {
+        bool ret = false;
+        visit(
+            [&](auto& bench)
+            {
+                [&]<std::size_t... I>(std::index_sequence<I...>)
+                {
+                    ((ret = std::get<I>(bench)->try_exchange(std::get<I>(ic).first,
+                                                             std::get<I>(ic).second,
+                                                             std::get<I>(oc)) ||
+                            ret),
+                     ...);
+                }(std::index_sequence_for<Ks...>{});
+            });
+        return ret;
+    }
+
+    /// @brief The outputs still held by every stream.
+    template <typename OutputCollection>
+    void
+    finalize(OutputCollection& oc)
+    {
+        visit(
+            [&](auto& bench)
+            {
+                [&]<std::size_t... I>(std::index_sequence<I...>)
+                {
+                    (std::ranges::copy(std::get<I>(bench)->stream.get_output_zipped(),
+                                       std::get<I>(oc)),
+                     ...);
+                }(std::index_sequence_for<Ks...>{});
+            });
+    }
+
+    std::size_t
+    workers() const noexcept
+    {
+        return m_benches.size();
+    }
+};
+
+/// @brief Input collection complete: every kernel's inputs were taken
+/// (reference mp::is_complete).
+template <typename InputCollection>
+bool
+is_complete(const InputCollection& ic) noexcept
+{
+    return std::apply([](const auto&... p) { return (... && (p.first == p.second)); }, ic);
+}
+
+template <Kernel K>
+using context = basic_context<hybrid_stream, K>;
+
+template <Kernel... Ks>
+using shop = basic_shop<hybrid_stream, Ks...>;
+
 } // namespace loom::vm::hip

@@ -76,67 +76,72 @@ pow_int(T x, long e) noexcept
 }
 
 /// @brief `s2f_fn` (string.hpp:312-352).
+///
+/// A literal reads "[sign] whole [. fraction] [e|E [sign] exponent]".  Its
+/// value is built in T as the reference builds it: whole as an integer
+/// converted to T, plus fraction-as-integer times 10^-(fraction digits),
+/// times 10^exponent, times the sign.  Where the reference finds the first
+/// '.' and the first 'e'/'E' of the whole text, this parser cuts the text
+/// at the exponent marker first and looks for '.' only before it; the
+/// accepted set is the same, because a '.' after the marker leaves the
+/// marker among the whole-part digits there and among the exponent digits
+/// here, and both parsers then reject the text.  Empty pieces are skipped:
+/// "e5" is 0 and "1e" is 1, as in the reference.
 template <typename T>
 constexpr std::optional<T>
-parse_float(std::string_view word) noexcept
+parse_float(std::string_view text) noexcept
 {
     T sign = static_cast<T>(1);
-    if(!word.empty() && (word[0] == '-' || word[0] == '+'))
+    if(!text.empty() && (text.front() == '-' || text.front() == '+'))
     {
-        if(word[0] == '-') sign = static_cast<T>(-1);
-        word = word.substr(1);
+        if(text.front() == '-') sign = static_cast<T>(-1);
+        text.remove_prefix(1);
     }
-    const std::string_view s = word;
-    if(s.empty()) return std::nullopt;
+    if(text.empty()) return std::nullopt;
 
-    const std::size_t sep_pos = s.find('.');
-    const std::size_t ep1     = s.find('e');
-    const std::size_t ep2     = s.find('E');
-    const std::size_t exp_pos = ep1 < ep2 ? ep1 : ep2;
-    const bool        has_sep = sep_pos != std::string_view::npos;
-    const bool        has_exp = exp_pos != std::string_view::npos;
+    std::size_t marker = 0;
+    while(marker < text.size() && text[marker] != 'e' && text[marker] != 'E') ++marker;
+    std::string_view       mantissa = text.substr(0, marker);
+    const std::string_view exponent =
+        marker < text.size() ? text.substr(marker + 1) : std::string_view{};
 
-    const std::string_view int_part =
-        has_sep ? s.substr(0, sep_pos) : (has_exp ? s.substr(0, exp_pos) : s);
-    const std::string_view frac_part =
-        has_sep ? (has_exp ? s.substr(sep_pos + 1, exp_pos - sep_pos - 1) :
-                             s.substr(sep_pos + 1)) :
-                  std::string_view{};
-    const std::string_view exp_part =
-        has_exp ? s.substr(exp_pos + 1) : std::string_view{};
-
-    T ret{};
-    if(!int_part.empty())
+    std::string_view fraction{};
+    if(const std::size_t dot = mantissa.find('.'); dot != std::string_view::npos)
     {
-        const auto oi = parse_digits(int_part, 10);
-        if(!oi) return std::nullopt;
-        ret = static_cast<T>(static_cast<long>(*oi));
+        fraction = mantissa.substr(dot + 1);
+        mantissa = mantissa.substr(0, dot);
     }
-    if(!frac_part.empty())
+
+    T value{};
+    if(!mantissa.empty())
     {
-        const auto oi = parse_digits(frac_part, 10);
-        if(!oi) return std::nullopt;
-        const T frac = static_cast<T>(static_cast<long>(*oi)) *
-                       pow_int<T>(static_cast<T>(10),
-                                  -static_cast<long>(static_cast<int>(
-                                      frac_part.length())));
-        ret += frac;
+        const auto digits = parse_digits(mantissa, 10);
+        if(!digits) return std::nullopt;
+        value = static_cast<T>(static_cast<long>(*digits));
     }
-    if(!exp_part.empty())
+    if(!fraction.empty())
     {
-        std::string_view e   = exp_part;
-        long             esg = 1;
-        if(e[0] == '-' || e[0] == '+')
+        const auto digits = parse_digits(fraction, 10);
+        if(!digits) return std::nullopt;
+        const long places = static_cast<long>(static_cast<int>(fraction.size()));
+        value += static_cast<T>(static_cast<long>(*digits)) *
+                 pow_int<T>(static_cast<T>(10), -places);
+    }
+    if(!exponent.empty())
+    {
+        std::string_view mag  = exponent;
+        long             sign_e = 1;
+        if(mag.front() == '-' || mag.front() == '+')
         {
-            if(e[0] == '-') esg = -1;
-            e = e.substr(1);
+            if(mag.front() == '-') sign_e = -1;
+            mag.remove_prefix(1);
         }
-        if(e.empty()) return std::nullopt;
-        const auto oi = parse_digits(e, 10);
-        if(!oi) return std::nullopt;
-        ret *= pow_int<T>(static_cast<T>(10), static_cast<long>(*oi) * esg);
+        if(mag.empty()) return std::nullopt;
+        const auto digits = parse_digits(mag, 10);
+        if(!digits) return std::nullopt;
+        value *= pow_int<T>(static_cast<T>(10), static_cast<long>(*digits) * sign_e);
     }
-    return ret * sign;
+    return value * sign;
 }
 
 /// @brief `s2i_base_prefix_fn` (string.hpp:286-306) for an integer of type
@@ -199,4 +204,28 @@ bits_of(T v) noexcept
     else
         return std::bit_cast<std::uint8_t>(v);
 }
+
+/// The reference's own checks of its float parser (src/tests/string.cpp:40-56):
+/// these texts parse to the value the compiler gives the same literal, and
+/// three texts are rejected.  Every translation unit that includes this
+/// header re-checks them at compile time.
+namespace checks
+{
+static_assert(parse_float<float>("0").value() == 0.0f);
+static_assert(parse_float<float>("1").value() == 1.0f);
+static_assert(parse_float<float>("1234.567").value() == 1234.567f);
+static_assert(parse_float<float>("1e2").value() == 1e2f);
+static_assert(parse_float<float>("0.5e2").value() == 0.5e2f);
+static_assert(parse_float<float>("0.5e-2").value() == 0.5e-2f);
+static_assert(parse_float<float>("1e-8").value() == 1e-8f);
+static_assert(parse_float<double>("0").value() == 0.0);
+static_assert(parse_float<double>("-1234.567").value() == -1234.567);
+static_assert(parse_float<double>("-1e2").value() == -1e2);
+static_assert(parse_float<double>("-0.5e2").value() == -0.5e2);
+static_assert(parse_float<double>("-0.5e-2").value() == -0.5e-2);
+static_assert(parse_float<double>("-2430.5e4").value() == -2430.5e4);
+static_assert(!parse_float<float>("e334g").has_value());
+static_assert(!parse_float<float>("-").has_value());
+static_assert(!parse_float<float>("+").has_value());
+} // namespace checks
 } // namespace loom::vm::literal

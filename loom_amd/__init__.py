@@ -114,10 +114,15 @@ class DeviceArray:
             abi.check(abi.load().loom_memcpy_h2d(d.ptr, a.ctypes.data, d.nbytes), "h2d")
         return d
 
-    def numpy(self) -> np.ndarray:
-        out = np.empty(self.n, self.dtype)
-        if self.n:
-            abi.check(abi.load().loom_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "d2h")
+    def numpy(self, begin: int = 0, end: int | None = None) -> np.ndarray:
+        """Elements [begin, end) copied to the host (the whole column by
+        default)."""
+        end = self.n if end is None else min(int(end), self.n)
+        begin = min(max(0, int(begin)), end)
+        out = np.empty(end - begin, self.dtype)
+        if out.size:
+            abi.check(abi.load().loom_memcpy_d2h(
+                out.ctypes.data, self.ptr + begin * self.dtype.itemsize, out.nbytes), "d2h")
         return out
 
     def zero(self) -> None:
@@ -184,12 +189,14 @@ def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, wa
     return p
 
 
-def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = None) -> RunResult:
+def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = None,
+        context: "Context | None" = None) -> RunResult:
     """Run every thread of `inputs` to HALT on the GPU.
 
     `inputs` are numpy arrays (copied in/out through the C ABI) or
     DeviceArray columns (then `outputs` must be DeviceArray columns too and
-    the run reads and writes HBM only)."""
+    the run reads and writes HBM only).  `context` is an execution context
+    of the calling thread (default: the library's default context)."""
     init()
     k = kernel(k)
     lib = abi.load()
@@ -197,6 +204,7 @@ def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = 
         raise ValueError(f"{k.name} takes {len(k.inputs)} inputs, got {len(inputs)}")
     params = params or make_params(mode=mode)
     st = abi.RunStats()
+    ctx = context.handle if context is not None else None
     device = isinstance(inputs[0], DeviceArray)
     if n is None:
         n = inputs[0].n if device else len(inputs[0])
@@ -208,8 +216,8 @@ def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = 
                 raise ValueError("device input column of wrong type or size")
         ins = (C.c_void_p * abi.MAX_IO)(*[a.ptr for a in inputs])
         outs = (C.c_void_p * abi.MAX_IO)(*[a.ptr for a in outputs])
-        abi.check(lib.loom_run(k.index, mode, ins, outs, n, C.byref(params), C.byref(st)),
-                  "loom_run")
+        abi.check(lib.loom_ctx_run(ctx, k.index, mode, ins, outs, n, C.byref(params),
+                                   C.byref(st)), "loom_run")
     else:
         host_in = [np.ascontiguousarray(a, dtype=dt) for a, dt in zip(inputs, k.inputs)]
         for a in host_in:
@@ -219,11 +227,44 @@ def run(k, mode: int, inputs, outputs=None, n=None, params: abi.Params | None = 
             outputs = [np.zeros(n, dt) for dt in k.outputs]
         ins = (C.c_void_p * abi.MAX_IO)(*[a.ctypes.data for a in host_in])
         outs = (C.c_void_p * abi.MAX_IO)(*[a.ctypes.data for a in outputs])
-        abi.check(lib.loom_run_host(k.index, mode, ins, outs, n, C.byref(params),
-                                    C.byref(st)), "loom_run_host")
+        abi.check(lib.loom_ctx_run_host(ctx, k.index, mode, ins, outs, n, C.byref(params),
+                                        C.byref(st)), "loom_run_host")
     return RunResult(outputs=outputs, ms_total=st.ms_total, ms_main=st.ms_main,
                      main_launches=st.main_launches, passes=st.passes, errors=st.errors,
                      grid_waves=st.grid_waves, counters=list(st.counters))
+
+
+class Context:
+    """An execution context for one host thread (loom_context: a kernel
+    stream and a copy stream of its own, its counters and workspaces).
+    Threads with contexts of their own run concurrently and their kernels
+    overlap on the GPU (the reference's Shop workers, src/vm/mp.hpp:158-190)."""
+
+    def __init__(self):
+        init()
+        h = C.c_void_p()
+        abi.check(abi.load().loom_context_create(C.byref(h)), "loom_context_create")
+        self.handle = h
+
+    def run(self, k, mode: int, inputs, outputs=None, n=None, params=None) -> RunResult:
+        return run(k, mode, inputs, outputs, n, params, context=self)
+
+    def close(self):
+        if self.handle:
+            abi.load().loom_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class compact_function:

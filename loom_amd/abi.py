@@ -127,6 +127,19 @@ class Request(C.Structure):
         ("done_ms", C.c_double),
         ("steps", C.c_uint32),
         ("status", C.c_int32),
+        ("complete", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class StreamCounters(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64),
+        ("exchanges", C.c_uint64),
+        ("rows_in", C.c_uint64),
+        ("rows_out", C.c_uint64),
+        ("bytes_h2d", C.c_uint64),
+        ("bytes_d2h", C.c_uint64),
     ]
 
 
@@ -190,7 +203,43 @@ SIGNATURES = {
     "loom_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "loom_server_run": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_uint64, C.c_int,
                                   C.POINTER(ServerStats)]),
+    "loom_server_start": (C.c_int, [C.c_void_p]),
+    "loom_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request)]),
+    "loom_server_poll": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int)]),
+    "loom_server_wait": (C.c_int, [C.c_void_p, C.POINTER(Request)]),
+    "loom_server_stop": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "loom_server_destroy": (C.c_int, [C.c_void_p]),
+    "loom_context_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "loom_context_destroy": (C.c_int, [C.c_void_p]),
+    "loom_ctx_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                               C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
+                               C.POINTER(RunStats)]),
+    "loom_ctx_run_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
+                                    C.POINTER(RunStats)]),
+    "loom_ctx_fill": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64]),
+    "loom_ctx_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Job), C.c_int,
+                                     C.POINTER(Params), C.POINTER(RunStats)]),
+    "loom_ctx_copy_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                      C.POINTER(C.c_uint64)]),
+    "loom_ctx_copy_status": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
+    "loom_ctx_copy_wait": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "loom_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "loom_stream_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(Params),
+                                     C.POINTER(C.c_void_p)]),
+    "loom_stream_destroy": (C.c_int, [C.c_void_p]),
+    "loom_stream_set_inputs": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint64,
+                                         C.POINTER(C.c_uint64)]),
+    "loom_stream_step": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "loom_stream_sort": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "loom_stream_exchange": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint64,
+                                       C.POINTER(C.c_void_p), C.c_void_p,
+                                       C.POINTER(C.c_uint64)]),
+    "loom_stream_drain": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64)]),
+    "loom_stream_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64)]),
+    "loom_stream_counters_get": (C.c_int, [C.c_void_p, C.POINTER(StreamCounters)]),
 }
 
 _lib = None

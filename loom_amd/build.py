@@ -46,7 +46,7 @@ COMMON = [
 
 def sources() -> list[Path]:
     csrc = PKG / "csrc"
-    return [csrc / "runtime.cpp", csrc / "serve.cpp"] + sorted(csrc.glob("kernels_*.hip"))
+    return [csrc / "runtime.cpp", csrc / "serve.cpp", csrc / "stream.cpp"] + sorted(csrc.glob("kernels_*.hip"))
 
 
 def _compile(src: Path) -> Path:
@@ -86,23 +86,6 @@ FACADE_SRC = ROOT / "tests" / "cpp" / "facade_test.cpp"
 FACADE_BIN = ROOT / "tests" / "cpp" / "_build" / "facade_test"
 
 
-CALIBRATE_VALU_SRC = PKG / "csrc" / "calibrate_valu.hip"
-CALIBRATE_VALU_BIN = BUILD / "calibrate_valu"
-
-
-def build_calibrate_valu() -> Path:
-    """Standalone VALU rate probe (reading the roofline of the VALU-bound
-    path); same code-generation flags as the library."""
-    BUILD.mkdir(parents=True, exist_ok=True)
-    tmp = CALIBRATE_VALU_BIN.with_suffix(".tmp")
-    cmd = [HIPCC, *COMMON, "-x", "hip", str(CALIBRATE_VALU_SRC), "-o", str(tmp)]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"calibrate_valu build failed\n{r.stderr[-8000:]}")
-    os.replace(tmp, CALIBRATE_VALU_BIN)
-    return CALIBRATE_VALU_BIN
-
-
 def build_facade_test() -> Path:
     """Host program that drives the library through the C++ facade
     (include/loom_vm/vm/hip.hpp); host compiler only, links the library."""
@@ -128,7 +111,6 @@ def main(argv=None) -> int:
     if not a.oracle:
         print("built", build_lib(a.j), flush=True)
         print("built", build_facade_test(), flush=True)
-        print("built", build_calibrate_valu(), flush=True)
     if not a.lib:
         print("built", build_oracle(), flush=True)
     return 0

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "device/batch.hpp"
+#include "device/stream.hpp"
 #include "device/trace.hpp"
 
 #include <loom_amd.h>
@@ -40,6 +41,12 @@ struct KernelEntry
     /// step trace of threads [0, n) (trace_kernel)
     hipError_t (*trace)(const IoArgs&, std::uint64_t n, std::uint32_t max_steps,
                         std::uint64_t* rows, std::uint32_t* steps, hipStream_t) = nullptr;
+    /// device-resident stream: one step over every live slot (per policy),
+    /// and the initialise / exchange / drain swap of slot rows
+    hipError_t (*stream_step[kNumPolicies])(const StreamArgs&, const CtlArgs&, hipStream_t) = {};
+    hipError_t (*stream_swap)(const StreamArgs&, const StreamIo&, hipStream_t) = nullptr;
+    /// register bytes of one stream slot (sum of the register widths)
+    std::uint32_t state_bytes = 0;
     /// name, first type, width, type count and LOOM_REG_* flags of register r
     /// (returns 0 when r < nr)
     int (*reg)(int r, const char** name, std::size_t* name_len, int* type, int* width,
@@ -167,6 +174,27 @@ kernel_attrs(int mode, int policy, bool regroup, int* vgprs, int* scratch)
     return 0;
 }
 
+template <typename K, int P>
+hipError_t
+launch_stream_step(const StreamArgs& a, const CtlArgs& c, hipStream_t s)
+{
+    if(a.slots == 0) return hipSuccess;
+    hipLaunchKernelGGL((stream_step_kernel<K, P>), dim3(static_cast<unsigned>(a.slots / 64)),
+                       dim3(64), 0, s, a, c);
+    return hipGetLastError();
+}
+
+template <typename K>
+hipError_t
+launch_stream_swap(const StreamArgs& a, const StreamIo& x, hipStream_t s)
+{
+    const std::uint64_t rows = x.mode == SWAP_INIT ? a.slots : x.m;
+    if(rows == 0) return hipSuccess;
+    hipLaunchKernelGGL((stream_swap_kernel<K>), dim3(static_cast<unsigned>((rows + 255) / 256)),
+                       dim3(256), 0, s, a, x);
+    return hipGetLastError();
+}
+
 template <typename K>
 hipError_t
 launch_trace(const IoArgs& io, std::uint64_t n, std::uint32_t max_steps, std::uint64_t* rows,
@@ -241,6 +269,11 @@ make_entry(const char* name)
     e.hybrid_occupancy[0] = &hybrid_occupancy<K, POLICY_SUCCESSOR_MAJOR>;
     e.hybrid_occupancy[1] = &hybrid_occupancy<K, POLICY_TOPO>;
     e.hybrid_occupancy[2] = &hybrid_occupancy<K, POLICY_SWEEP>;
+    e.stream_step[0]      = &launch_stream_step<K, POLICY_SUCCESSOR_MAJOR>;
+    e.stream_step[1]      = &launch_stream_step<K, POLICY_TOPO>;
+    e.stream_step[2]      = &launch_stream_step<K, POLICY_SWEEP>;
+    e.stream_swap         = &launch_stream_swap<K>;
+    e.state_bytes         = StateLayout<K>::bytes;
     e.kernel_attrs     = &kernel_attrs<K>;
     e.dump             = &dump_of<K>;
     e.trace            = &launch_trace<K>;

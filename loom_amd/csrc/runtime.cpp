@@ -1,23 +1,25 @@
 // -*- C++ -*-
 /// @file runtime.cpp
 /// @brief The C ABI (include/loom_amd.h): device set-up, kernel registry,
-/// device memory, and the compact / hybrid run drivers.
+/// device memory, execution contexts, and the compact / hybrid / batched run
+/// drivers.
 ///
-/// One HIP stream per process carries all work (default priority).  Hybrid
-/// runs keep a grow-only workspace (control block and two queue sides) per
-/// kernel, so repeated runs do not allocate.
-#include "registry.hpp"
-
-#include <hip/hip_runtime.h>
+/// Work is issued on execution contexts (loom_context): each owns a kernel
+/// stream and a copy stream (default priority), its error flags and
+/// counters, and grow-only workspaces, so repeated runs do not allocate.
+/// A context's lock is held for the whole of each call on it, so one
+/// context may be shared by several host threads (their calls serialise),
+/// and host threads with contexts of their own run concurrently.  The entry
+/// points without a context argument use the default context that
+/// loom_init creates.
+#include "runtime.hpp"
 
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <deque>
-#include <mutex>
+#include <memory>
 #include <string>
-#include <vector>
 
 namespace loom::hipvm
 {
@@ -39,99 +41,17 @@ batch_registry()
 
 thread_local std::string g_error;
 
-int
-fail(int code, const char* fmt, ...)
+Device        g_dev;
+std::mutex    g_init_mutex;
+loom_context* g_default = nullptr;
+
+IoArgs
+make_io(const KernelEntry& e, const void* const* in, void* const* out)
 {
-    char    buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_error = buf;
-    return code;
-}
-
-#define LOOM_HIP(expr)                                                              \
-    do                                                                              \
-    {                                                                               \
-        const hipError_t e_ = (expr);                                               \
-        if(e_ != hipSuccess)                                                        \
-            return fail(static_cast<int>(e_), "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while(0)
-
-struct Device
-{
-    bool          ready = false;
-    int           id    = 0;
-    int           cus   = 0;
-    hipStream_t   stream{};
-    std::uint32_t* err   = nullptr; // device error flags
-    std::uint64_t* stats = nullptr; // device counters
-    std::uint32_t* host_err = nullptr;   // pinned
-    std::uint64_t* host_stats = nullptr; // pinned
-    HybridCtl*     host_ctl = nullptr;   // pinned mirror of a control block
-    hipEvent_t     ev[4]{};
-    std::vector<hipEvent_t> pass_ev;
-    JobDesc*       jobs = nullptr; // device job table of direct batch runs
-    BatchCtl*      bctl = nullptr;
-    // background copies: a second stream (default priority) and the events
-    // of outstanding tickets, oldest first (copies complete in order)
-    hipStream_t    copy{};
-    std::deque<std::pair<std::uint64_t, hipEvent_t>> tickets;
-    std::vector<hipEvent_t> spare_events;
-    std::uint64_t  last_ticket = 0;
-};
-
-Device g_dev;
-std::mutex g_mutex;
-
-struct Workspace
-{
-    HybridCtl*    ctl   = nullptr;
-    char*         q[2]  = {nullptr, nullptr};
-    std::size_t   bytes = 0; // per side
-};
-
-std::vector<Workspace> g_ws; // per kernel index
-
-struct HostCols
-{
-    std::vector<void*>       ptr;
-    std::vector<std::size_t> bytes;
-};
-HostCols g_hostcols;
-
-int
-ensure_device()
-{
-    if(!g_dev.ready) return fail(LOOM_E_NODEVICE, "loom_init has not been called");
-    return 0;
-}
-
-int
-type_size(int t)
-{
-    switch(t)
-    {
-    case LOOM_U8:
-    case LOOM_I8: return 1;
-    case LOOM_U16:
-    case LOOM_I16: return 2;
-    case LOOM_U32:
-    case LOOM_I32:
-    case LOOM_F32: return 4;
-    default: return 8;
-    }
-}
-
-int
-validate_params(const loom_params* params)
-{
-    if(params && (params->threshold > 63 || params->live_threshold > 63))
-        return fail(LOOM_E_ARG, "thresholds must be < 64");
-    if(params && params->reconv >= static_cast<std::uint32_t>(kNumPolicies))
-        return fail(LOOM_E_ARG, "unknown reconvergence policy");
-    return 0;
+    IoArgs io{};
+    for(int k = 0; k < e.info.n_inputs; ++k) io.in[k] = in[k];
+    for(int k = 0; k < e.info.n_outputs; ++k) io.out[k] = out[k];
+    return io;
 }
 
 int
@@ -149,57 +69,89 @@ validate_run(const KernelEntry& e, int mode, const void* const* inputs, void* co
     return validate_params(params);
 }
 
+/// @brief Resolve a context argument: NULL means the default context.
 int
-check_kernel(int k)
+context_of(loom_context* ctx, loom_context*& out)
 {
-    if(k < 0 || k >= static_cast<int>(registry().size()))
-        return fail(LOOM_E_KERNEL, "unknown kernel index %d", k);
+    if(int r = ensure_device()) return r;
+    out = ctx ? ctx : g_default;
+    if(!out) return fail(LOOM_E_NODEVICE, "no default context");
     return 0;
 }
 
-IoArgs
-make_io(const KernelEntry& e, const void* const* in, void* const* out)
+int
+context_init(loom_context& x)
 {
-    IoArgs io{};
-    for(int k = 0; k < e.info.n_inputs; ++k) io.in[k] = in[k];
-    for(int k = 0; k < e.info.n_outputs; ++k) io.out[k] = out[k];
-    return io;
+    LOOM_HIP(hipSetDevice(g_dev.id));
+    LOOM_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    LOOM_HIP(hipStreamCreateWithFlags(&x.copy, hipStreamNonBlocking));
+    LOOM_HIP(hipMalloc(&x.err, sizeof(std::uint32_t)));
+    LOOM_HIP(hipMalloc(&x.stats, sizeof(std::uint64_t) * kStatWords));
+    LOOM_HIP(hipHostMalloc(&x.host_err, sizeof(std::uint32_t), hipHostMallocDefault));
+    LOOM_HIP(hipHostMalloc(&x.host_stats, sizeof(std::uint64_t) * kStatWords, hipHostMallocDefault));
+    LOOM_HIP(hipHostMalloc(&x.host_ctl, sizeof(HybridCtl), hipHostMallocDefault));
+    for(auto& ev: x.ev) LOOM_HIP(hipEventCreate(&ev));
+    return 0;
+}
+
+/// @brief Release everything a context holds (after its work finished).
+void
+context_release(loom_context& x)
+{
+    if(x.stream) (void)hipStreamSynchronize(x.stream);
+    if(x.copy) (void)hipStreamSynchronize(x.copy);
+    for(auto& w: x.ws)
+    {
+        if(w.ctl) (void)hipFree(w.ctl);
+        for(auto q: w.q)
+            if(q) (void)hipFree(q);
+    }
+    for(void* p: x.hostcol_ptr)
+        if(p) (void)hipFree(p);
+    if(x.jobs) (void)hipFree(x.jobs);
+    if(x.bctl) (void)hipFree(x.bctl);
+    for(auto& t: x.tickets) (void)hipEventDestroy(t.second);
+    for(auto e: x.spare_events) (void)hipEventDestroy(e);
+    for(auto e: x.pass_ev) (void)hipEventDestroy(e);
+    for(auto e: x.ev)
+        if(e) (void)hipEventDestroy(e);
+    if(x.err) (void)hipFree(x.err);
+    if(x.stats) (void)hipFree(x.stats);
+    if(x.host_err) (void)hipHostFree(x.host_err);
+    if(x.host_stats) (void)hipHostFree(x.host_stats);
+    if(x.host_ctl) (void)hipHostFree(x.host_ctl);
+    if(x.stream) (void)hipStreamDestroy(x.stream);
+    if(x.copy) (void)hipStreamDestroy(x.copy);
 }
 
 int
-finish_stats(loom_run_stats* st, const loom_params* p)
+finish_stats(loom_context& x, loom_run_stats* st, const loom_params* p)
 {
-    LOOM_HIP(hipMemcpyAsync(g_dev.host_err, g_dev.err, sizeof(std::uint32_t),
-                            hipMemcpyDeviceToHost, g_dev.stream));
+    LOOM_HIP(hipMemcpyAsync(x.host_err, x.err, sizeof(std::uint32_t), hipMemcpyDeviceToHost,
+                            x.stream));
     if(p && p->collect_stats)
-        LOOM_HIP(hipMemcpyAsync(g_dev.host_stats, g_dev.stats, sizeof(std::uint64_t) * kStatWords,
-                                hipMemcpyDeviceToHost, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+        LOOM_HIP(hipMemcpyAsync(x.host_stats, x.stats, sizeof(std::uint64_t) * kStatWords,
+                                hipMemcpyDeviceToHost, x.stream));
+    LOOM_HIP(hipStreamSynchronize(x.stream));
     if(st)
     {
-        st->errors = *g_dev.host_err;
-        if(p && p->collect_stats)
-            for(std::size_t i = 0; i < kNumStats; ++i)
-            {
-                std::uint64_t sum = 0;
-                for(std::size_t k = 0; k < kStatSlots; ++k) sum += g_dev.host_stats[k * kNumStats + i];
-                st->counters[i] = sum;
-            }
+        st->errors = *x.host_err;
+        if(p && p->collect_stats) sum_counters(x.host_stats, st->counters);
     }
-    if(*g_dev.host_err)
-        return fail(LOOM_E_RUNTIME, "kernel raised error flags 0x%x", *g_dev.host_err);
+    if(*x.host_err) return fail(LOOM_E_RUNTIME, "kernel raised error flags 0x%x", *x.host_err);
     return 0;
 }
 
 int
-run_compact(const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy, loom_run_stats* st)
+run_compact(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy,
+            loom_run_stats* st)
 {
-    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    LOOM_HIP(e.compact[policy](io, c, g_dev.stream));
-    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
-    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
+    LOOM_HIP(e.compact[policy](io, c, x.stream));
+    LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
+    LOOM_HIP(hipEventSynchronize(x.ev[1]));
     float ms = 0;
-    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
+    LOOM_HIP(hipEventElapsedTime(&ms, x.ev[0], x.ev[1]));
     if(st)
     {
         st->ms_total      = ms;
@@ -211,46 +163,39 @@ run_compact(const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy, loom_
     return 0;
 }
 
-int
-run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const loom_params* p,
-           loom_run_stats* st)
+/// @brief The persistent grid of a hybrid run: resident 64-lane workgroups
+/// (waves_per_cu per CU, capped by occupancy and by the work).
+unsigned
+hybrid_grid(const KernelEntry& e, int policy, bool regroup, const loom_params* p, std::uint64_t n)
 {
-    // persistent grid: resident 64-lane workgroups
-    const int policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
-    const int occ    = e.hybrid_occupancy[policy](c.regroup != 0);
+    const int occ = e.hybrid_occupancy[policy](regroup);
     int       wpc = p && p->waves_per_cu ? static_cast<int>(p->waves_per_cu) : occ;
     wpc           = std::max(1, std::min(wpc, occ));
     unsigned grid = static_cast<unsigned>(g_dev.cus * wpc);
-    const std::uint64_t need_waves = (c.n + 63) / 64;
+    const std::uint64_t need_waves = (n + 63) / 64;
     if(need_waves < grid) grid = static_cast<unsigned>(std::max<std::uint64_t>(1, need_waves));
+    return grid;
+}
 
-    // queue capacity: rows per label, a multiple of 64
+/// @brief Queue capacity (rows per label, a multiple of 64) of a hybrid run.
+std::uint64_t
+hybrid_qcap(const loom_params* p, unsigned grid, std::uint64_t n)
+{
     std::uint64_t qcap = p && p->queue_capacity ? p->queue_capacity :
                                                   std::uint64_t(grid) * 64ull * 4ull;
-    qcap = std::min<std::uint64_t>(qcap, std::max<std::uint64_t>(c.n, 64));
-    qcap = (qcap + 63) / 64 * 64;
-    if(qcap > 0xFFFFFFC0ull) return fail(LOOM_E_CAPACITY, "queue capacity too large");
+    qcap = std::min<std::uint64_t>(qcap, std::max<std::uint64_t>(n, 64));
+    return (qcap + 63) / 64 * 64;
+}
 
-    if(g_ws.size() < registry().size()) g_ws.resize(registry().size());
-    Workspace&        ws   = g_ws[kidx];
-    const std::size_t side = std::size_t(qcap) * e.info.queue_row_bytes;
-    if(!ws.ctl) LOOM_HIP(hipMalloc(&ws.ctl, sizeof(HybridCtl)));
-    if(ws.bytes < side)
-    {
-        for(auto& q: ws.q)
-            if(q) LOOM_HIP(hipFree(q));
-        for(auto& q: ws.q) LOOM_HIP(hipMalloc(&q, side));
-        ws.bytes = side;
-    }
-    HybridArgs h;
-    h.ctl  = ws.ctl;
-    h.q[0] = ws.q[0];
-    h.q[1] = ws.q[1];
-    h.qcap = qcap;
-
+/// @brief Launch sequence of a hybrid run over the workspace @p h (control
+/// block and queue sides) on @p x's stream.
+int
+run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy,
+              unsigned grid, const HybridArgs& h, loom_run_stats* st)
+{
     const std::uint32_t nb = static_cast<std::uint32_t>(e.info.n_blocks);
-    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream, ws.ctl, nb, qcap,
+    LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
+    hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, x.stream, h.ctl, nb, h.qcap,
                        c.n, 1u);
     LOOM_HIP(hipGetLastError());
 
@@ -260,13 +205,13 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
     std::size_t ev_used = 0;
     auto next_ev = [&]() -> hipEvent_t
     {
-        if(ev_used == g_dev.pass_ev.size())
+        if(ev_used == x.pass_ev.size())
         {
             hipEvent_t ev{};
             if(hipEventCreate(&ev) != hipSuccess) return nullptr;
-            g_dev.pass_ev.push_back(ev);
+            x.pass_ev.push_back(ev);
         }
-        return g_dev.pass_ev[ev_used++];
+        return x.pass_ev[ev_used++];
     };
     if(!c.regroup)
     {
@@ -276,12 +221,12 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
         // read-back and host round trip is needed to learn that it is done.
         hipEvent_t a = next_ev(), b = next_ev();
         if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
-        LOOM_HIP(hipEventRecord(a, g_dev.stream));
-        LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
-        LOOM_HIP(hipEventRecord(b, g_dev.stream));
+        LOOM_HIP(hipEventRecord(a, x.stream));
+        LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, x.stream));
+        LOOM_HIP(hipEventRecord(b, x.stream));
         evs.emplace_back(a, b);
         passes = 1;
-        LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
+        LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
     }
     else
         for(;;)
@@ -290,29 +235,29 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
             {
                 hipEvent_t a = next_ev(), b = next_ev();
                 if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
-                LOOM_HIP(hipEventRecord(a, g_dev.stream));
-                LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, g_dev.stream));
-                LOOM_HIP(hipEventRecord(b, g_dev.stream));
+                LOOM_HIP(hipEventRecord(a, x.stream));
+                LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, x.stream));
+                LOOM_HIP(hipEventRecord(b, x.stream));
                 evs.emplace_back(a, b);
-                hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, g_dev.stream,
-                                   ws.ctl, nb, qcap, c.n, 0u);
+                hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, x.stream, h.ctl, nb,
+                                   h.qcap, c.n, 0u);
                 LOOM_HIP(hipGetLastError());
                 ++passes;
             }
-            LOOM_HIP(hipMemcpyAsync(g_dev.host_ctl, ws.ctl, sizeof(HybridCtl),
-                                    hipMemcpyDeviceToHost, g_dev.stream));
+            LOOM_HIP(hipMemcpyAsync(x.host_ctl, h.ctl, sizeof(HybridCtl), hipMemcpyDeviceToHost,
+                                    x.stream));
             // the end of the run, if this check finds it done: recorded
             // before the host waits, so the host's round trip is not timed
-            LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
-            LOOM_HIP(hipStreamSynchronize(g_dev.stream));
-            if(g_dev.host_ctl->done) break;
+            LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
+            LOOM_HIP(hipStreamSynchronize(x.stream));
+            if(x.host_ctl->done) break;
             if(passes >= max_passes)
                 return fail(LOOM_E_RUNTIME, "hybrid run exceeded %u passes", max_passes);
             check_every = std::min<std::uint32_t>(check_every * 2, 8);
         }
-    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    LOOM_HIP(hipEventSynchronize(x.ev[1]));
     float total = 0;
-    LOOM_HIP(hipEventElapsedTime(&total, g_dev.ev[0], g_dev.ev[1]));
+    LOOM_HIP(hipEventElapsedTime(&total, x.ev[0], x.ev[1]));
     double main_ms = 0;
     for(auto& [a, b]: evs)
     {
@@ -330,6 +275,103 @@ run_hybrid(int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c, const lo
     }
     return 0;
 }
+
+int
+run_hybrid(loom_context& x, int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c,
+           const loom_params* p, loom_run_stats* st)
+{
+    const int      policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
+    const unsigned grid   = hybrid_grid(e, policy, c.regroup != 0, p, c.n);
+    const std::uint64_t qcap = hybrid_qcap(p, grid, c.n);
+    if(qcap > 0xFFFFFFC0ull) return fail(LOOM_E_CAPACITY, "queue capacity too large");
+
+    if(x.ws.size() < registry().size()) x.ws.resize(registry().size());
+    auto&             ws   = x.ws[kidx];
+    const std::size_t side = std::size_t(qcap) * e.info.queue_row_bytes;
+    if(!ws.ctl) LOOM_HIP(hipMalloc(&ws.ctl, sizeof(HybridCtl)));
+    // queues are read only by runs that regroup
+    if(c.regroup && ws.bytes < side)
+    {
+        for(auto& q: ws.q)
+            if(q)
+            {
+                LOOM_HIP(hipFree(q));
+                q = nullptr;
+            }
+        ws.bytes = 0;
+        for(auto& q: ws.q) LOOM_HIP(hipMalloc(&q, side));
+        ws.bytes = side;
+    }
+    HybridArgs h;
+    h.ctl  = ws.ctl;
+    h.q[0] = ws.q[0];
+    h.q[1] = ws.q[1];
+    h.qcap = qcap;
+    return run_hybrid_on(x, e, io, c, policy, grid, h, st);
+}
+
+int
+run_on(loom_context& x, int kernel, int mode, const void* const* inputs, void* const* outputs,
+       std::uint64_t n, const loom_params* params, loom_run_stats* stats)
+{
+    if(int r = check_kernel(kernel)) return r;
+    const KernelEntry& e = registry()[kernel];
+    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
+    if(stats) *stats = loom_run_stats{};
+    const IoArgs  io = n ? make_io(e, inputs, outputs) : IoArgs{};
+    const CtlArgs c  = make_ctl(params, n, mode, x.err, x.stats);
+    LOOM_HIP(hipMemsetAsync(x.err, 0, sizeof(std::uint32_t), x.stream));
+    if(c.stats) LOOM_HIP(hipMemsetAsync(x.stats, 0, sizeof(std::uint64_t) * kStatWords, x.stream));
+    int r = 0;
+    if(mode == LOOM_MODE_COMPACT)
+        r = run_compact(x, e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
+    else
+        r = run_hybrid(x, kernel, e, io, c, params, stats);
+    if(r) return r;
+    return finish_stats(x, stats, params);
+}
+
+int
+run_host_on(loom_context& x, int kernel, int mode, const void* const* inputs,
+            void* const* outputs, std::uint64_t n, const loom_params* params,
+            loom_run_stats* stats)
+{
+    if(int r = check_kernel(kernel)) return r;
+    const KernelEntry& e = registry()[kernel];
+    // validate before allocating or copying anything
+    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
+    const int          ni = e.info.n_inputs, no = e.info.n_outputs;
+    std::vector<void*> din(ni), dout(no);
+    if(x.hostcol_ptr.size() < static_cast<std::size_t>(ni + no))
+    {
+        x.hostcol_ptr.resize(ni + no, nullptr);
+        x.hostcol_bytes.resize(ni + no, 0);
+    }
+    for(int k = 0; k < ni + no; ++k)
+    {
+        const int         t     = k < ni ? e.info.input_type[k] : e.info.output_type[k - ni];
+        const std::size_t bytes = std::max<std::size_t>(1, n * type_size(t));
+        if(x.hostcol_bytes[k] < bytes)
+        {
+            if(x.hostcol_ptr[k]) LOOM_HIP(hipFree(x.hostcol_ptr[k]));
+            x.hostcol_ptr[k]   = nullptr;
+            x.hostcol_bytes[k] = 0;
+            LOOM_HIP(hipMalloc(&x.hostcol_ptr[k], bytes));
+            x.hostcol_bytes[k] = bytes;
+        }
+        (k < ni ? din[k] : dout[k - ni]) = x.hostcol_ptr[k];
+    }
+    for(int k = 0; k < ni && n; ++k)
+        LOOM_HIP(hipMemcpyAsync(din[k], inputs[k], n * type_size(e.info.input_type[k]),
+                                hipMemcpyHostToDevice, x.stream));
+    if(int r = run_on(x, kernel, mode, din.data(), dout.data(), n, params, stats)) return r;
+    for(int k = 0; k < no && n; ++k)
+        LOOM_HIP(hipMemcpyAsync(outputs[k], dout[k], n * type_size(e.info.output_type[k]),
+                                hipMemcpyDeviceToHost, x.stream));
+    LOOM_HIP(hipStreamSynchronize(x.stream));
+    return 0;
+}
+
 /// @brief Broadcast one element over a column (grid-stride, vector stores).
 template <typename T>
 __global__ void
@@ -342,13 +384,13 @@ fill_kernel(T* dst, T value, std::uint64_t count)
 
 template <typename T>
 hipError_t
-launch_fill(void* dst, const void* pattern, std::uint64_t count)
+launch_fill(void* dst, const void* pattern, std::uint64_t count, hipStream_t s)
 {
     T v;
     std::memcpy(&v, pattern, sizeof(T));
     const std::uint64_t blocks = std::min<std::uint64_t>((count + 255) / 256, 4096);
-    hipLaunchKernelGGL(fill_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
-                       g_dev.stream, static_cast<T*>(dst), v, count);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                       static_cast<T*>(dst), v, count);
     return hipGetLastError();
 }
 
@@ -398,21 +440,6 @@ batch_jobs(int set, const loom_job* jobs, int njobs, std::vector<JobDesc>& out)
     return 0;
 }
 
-CtlArgs
-make_ctl(const loom_params* p, std::uint64_t n)
-{
-    CtlArgs c;
-    c.n         = n;
-    c.max_steps = p && p->max_steps ? p->max_steps : (1ull << 40);
-    c.thr       = p ? p->threshold : 0;
-    c.live_thr  = p ? p->live_threshold : 48;
-    c.refill    = p ? p->refill : 1;
-    c.regroup   = p ? p->regroup : 0;
-    c.err       = g_dev.err;
-    c.stats     = p && p->collect_stats ? g_dev.stats : nullptr;
-    return c;
-}
-
 int
 batch_policy(const loom_params* p)
 {
@@ -431,7 +458,193 @@ batch_grid(const BatchEntry& b, const loom_params* p, std::uint64_t threads)
     return grid;
 }
 
+int
+batch_run_on(loom_context& x, int set, const loom_job* jobs, int njobs, const loom_params* params,
+             loom_run_stats* stats)
+{
+    if(int r = validate_params(params)) return r;
+    std::vector<JobDesc> jd;
+    if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
+    if(stats) *stats = loom_run_stats{};
+    if(!x.jobs) LOOM_HIP(hipMalloc(&x.jobs, sizeof(JobDesc) * kMaxJobs));
+    if(!x.bctl) LOOM_HIP(hipMalloc(&x.bctl, sizeof(BatchCtl)));
+    LOOM_HIP(hipMemcpyAsync(x.jobs, jd.data(), sizeof(JobDesc) * jd.size(), hipMemcpyHostToDevice,
+                            x.stream));
+    std::uint64_t threads = 0;
+    for(const auto& j: jd) threads += j.end - j.begin;
+    const BatchEntry& b = batch_registry()[set];
+    const CtlArgs     c = make_ctl(params, threads, LOOM_MODE_HYBRID, x.err, x.stats);
+    BatchArgs         a;
+    a.jobs  = x.jobs;
+    a.ctl   = x.bctl;
+    a.njobs = static_cast<std::uint32_t>(njobs);
+    const unsigned grid = batch_grid(b, params, threads);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, x.stream, x.err, x.stats,
+                       static_cast<std::uint32_t>(kStatWords));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, x.stream, a);
+    LOOM_HIP(hipGetLastError());
+    LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
+    LOOM_HIP(b.launch[batch_policy(params)](c, a, grid, x.stream));
+    LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
+    LOOM_HIP(hipEventSynchronize(x.ev[1]));
+    float ms = 0;
+    LOOM_HIP(hipEventElapsedTime(&ms, x.ev[0], x.ev[1]));
+    if(stats)
+    {
+        stats->ms_total      = ms;
+        stats->ms_main       = ms;
+        stats->main_launches = 1;
+        stats->passes        = 1;
+        stats->grid_waves    = grid;
+    }
+    return finish_stats(x, stats, params);
+}
+
+int
+copy_async_on(loom_context& x, void* dst, const void* src, size_t bytes, int kind,
+              std::uint64_t* ticket)
+{
+    if(!ticket) return fail(LOOM_E_ARG, "null ticket");
+    if(bytes && (!dst || !src)) return fail(LOOM_E_ARG, "null pointer");
+    if(kind != LOOM_COPY_H2D && kind != LOOM_COPY_D2H)
+        return fail(LOOM_E_ARG, "unknown copy kind %d", kind);
+    if(bytes)
+        LOOM_HIP(hipMemcpyAsync(dst, src, bytes,
+                                kind == LOOM_COPY_H2D ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                                x.copy));
+    hipEvent_t ev{};
+    if(!x.spare_events.empty())
+    {
+        ev = x.spare_events.back();
+        x.spare_events.pop_back();
+    }
+    else
+        LOOM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    LOOM_HIP(hipEventRecord(ev, x.copy));
+    *ticket = ++x.last_ticket;
+    x.tickets.emplace_back(*ticket, ev);
+    return 0;
+}
+
+int
+copy_status_on(loom_context& x, std::uint64_t ticket, int* done)
+{
+    if(!done) return fail(LOOM_E_ARG, "null result");
+    while(!x.tickets.empty())
+    {
+        const hipError_t q = hipEventQuery(x.tickets.front().second);
+        if(q == hipErrorNotReady) break;
+        if(q != hipSuccess) return fail(static_cast<int>(q), "copy: %s", hipGetErrorString(q));
+        x.spare_events.push_back(x.tickets.front().second);
+        x.tickets.pop_front();
+    }
+    *done = x.tickets.empty() || x.tickets.front().first > ticket;
+    return 0;
+}
 } // namespace
+
+const std::vector<KernelEntry>&
+kernels()
+{
+    return registry();
+}
+
+const std::vector<BatchEntry>&
+batch_sets()
+{
+    return batch_registry();
+}
+
+const Device&
+device()
+{
+    return g_dev;
+}
+
+loom_context*
+default_context()
+{
+    return g_default;
+}
+
+int
+fail(int code, const char* fmt, ...)
+{
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int
+ensure_device()
+{
+    if(!g_dev.ready) return fail(LOOM_E_NODEVICE, "loom_init has not been called");
+    return 0;
+}
+
+int
+check_kernel(int k)
+{
+    if(k < 0 || k >= static_cast<int>(registry().size()))
+        return fail(LOOM_E_KERNEL, "unknown kernel index %d", k);
+    return 0;
+}
+
+int
+type_size(int t)
+{
+    switch(t)
+    {
+    case LOOM_U8:
+    case LOOM_I8: return 1;
+    case LOOM_U16:
+    case LOOM_I16: return 2;
+    case LOOM_U32:
+    case LOOM_I32:
+    case LOOM_F32: return 4;
+    default: return 8;
+    }
+}
+
+int
+validate_params(const loom_params* params)
+{
+    if(params && (params->threshold > 63 || params->live_threshold > 63))
+        return fail(LOOM_E_ARG, "thresholds must be < 64");
+    if(params && params->reconv >= static_cast<std::uint32_t>(kNumPolicies))
+        return fail(LOOM_E_ARG, "unknown reconvergence policy");
+    return 0;
+}
+
+CtlArgs
+make_ctl(const loom_params* p, std::uint64_t n, int mode, std::uint32_t* err, std::uint64_t* stats)
+{
+    CtlArgs c;
+    c.n         = n;
+    c.max_steps = p && p->max_steps ? p->max_steps : (1ull << 40);
+    c.thr       = p ? p->threshold : 0;
+    c.live_thr  = p ? p->live_threshold : (mode == LOOM_MODE_HYBRID ? 48 : 0);
+    c.refill    = p ? p->refill : 1;
+    c.regroup   = p ? p->regroup : 0;
+    c.err       = err;
+    c.stats     = p && p->collect_stats ? stats : nullptr;
+    return c;
+}
+
+void
+sum_counters(const std::uint64_t* host_stats, std::uint64_t* out)
+{
+    for(std::size_t i = 0; i < kNumStats; ++i)
+    {
+        std::uint64_t sum = 0;
+        for(std::size_t k = 0; k < kStatSlots; ++k) sum += host_stats[k * kNumStats + i];
+        out[i] = sum;
+    }
+}
 
 bool
 register_kernel(const KernelEntry& e)
@@ -450,6 +663,25 @@ register_batch(const BatchEntry& e)
 
 using namespace loom::hipvm;
 
+namespace
+{
+/// @brief Holds a context's lock for the rest of a call.
+struct Locked
+{
+    loom_context*                x = nullptr;
+    std::unique_lock<std::mutex> lk;
+};
+
+/// @brief Resolve @p ctx (NULL: the default context) and lock it.
+int
+lock_context(loom_context* ctx, Locked& l)
+{
+    if(int r = context_of(ctx, l.x)) return r;
+    l.lk = std::unique_lock<std::mutex>(l.x->mu);
+    return 0;
+}
+} // namespace
+
 extern "C" {
 
 const char*
@@ -461,7 +693,7 @@ loom_last_error(void)
 int
 loom_init(int device)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
+    std::lock_guard<std::mutex> lk(g_init_mutex);
     if(g_dev.ready) return device == g_dev.id ? 0 : fail(LOOM_E_ARG, "already initialised");
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -472,15 +704,42 @@ loom_init(int device)
     LOOM_HIP(hipGetDeviceProperties(&prop, device));
     g_dev.id  = device;
     g_dev.cus = prop.multiProcessorCount;
-    LOOM_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
-    LOOM_HIP(hipStreamCreateWithFlags(&g_dev.copy, hipStreamNonBlocking));
-    LOOM_HIP(hipMalloc(&g_dev.err, sizeof(std::uint32_t)));
-    LOOM_HIP(hipMalloc(&g_dev.stats, sizeof(std::uint64_t) * kStatWords));
-    LOOM_HIP(hipHostMalloc(&g_dev.host_err, sizeof(std::uint32_t), hipHostMallocDefault));
-    LOOM_HIP(hipHostMalloc(&g_dev.host_stats, sizeof(std::uint64_t) * kStatWords, hipHostMallocDefault));
-    LOOM_HIP(hipHostMalloc(&g_dev.host_ctl, sizeof(HybridCtl), hipHostMallocDefault));
-    for(auto& ev: g_dev.ev) LOOM_HIP(hipEventCreate(&ev));
+    auto x    = std::make_unique<loom_context>();
+    if(int r = context_init(*x))
+    {
+        context_release(*x);
+        return r;
+    }
+    g_default   = x.release();
     g_dev.ready = true;
+    return 0;
+}
+
+int
+loom_context_create(loom_context** ctx)
+{
+    if(int r = ensure_device()) return r;
+    if(!ctx) return fail(LOOM_E_ARG, "null context");
+    auto x = std::make_unique<loom_context>();
+    if(int r = context_init(*x))
+    {
+        context_release(*x);
+        return r;
+    }
+    *ctx = x.release();
+    return 0;
+}
+
+int
+loom_context_destroy(loom_context* ctx)
+{
+    if(!ctx) return 0;
+    if(ctx == g_default) return fail(LOOM_E_ARG, "the default context is not destroyed");
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        context_release(*ctx);
+    }
+    delete ctx;
     return 0;
 }
 
@@ -569,8 +828,8 @@ int
 loom_kernel_trace(int kernel, const void* const* inputs, uint64_t n, uint32_t max_steps,
                   uint64_t* rows, size_t rows_bytes, uint32_t* steps)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
     if(int r = check_kernel(kernel)) return r;
     const KernelEntry& e = registry()[kernel];
     if(n >= (1ull << 20)) return fail(LOOM_E_CAPACITY, "at most 2^20-1 traced threads");
@@ -588,8 +847,8 @@ loom_kernel_trace(int kernel, const void* const* inputs, uint64_t n, uint32_t ma
                     static_cast<unsigned long long>(need), rows_bytes);
     IoArgs io{};
     for(int k = 0; k < e.info.n_inputs; ++k) io.in[k] = inputs[k];
-    LOOM_HIP(e.trace(io, n, max_steps, rows, steps, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    LOOM_HIP(e.trace(io, n, max_steps, rows, steps, l.x->stream));
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
     return 0;
 }
 
@@ -623,55 +882,67 @@ loom_free(void* ptr)
 int
 loom_memcpy_h2d(void* dst, const void* src, size_t bytes)
 {
-    if(int r = ensure_device()) return r;
-    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, l.x->stream));
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
     return 0;
 }
 
 int
 loom_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {
-    if(int r = ensure_device()) return r;
-    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, l.x->stream));
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
     return 0;
 }
 
 int
 loom_memcpy_d2d(void* dst, const void* src, size_t bytes)
 {
-    if(int r = ensure_device()) return r;
-    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    LOOM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, l.x->stream));
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
     return 0;
 }
 
 int
 loom_memset(void* dst, int value, size_t bytes)
 {
-    if(int r = ensure_device()) return r;
-    LOOM_HIP(hipMemsetAsync(dst, value, bytes, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    LOOM_HIP(hipMemsetAsync(dst, value, bytes, l.x->stream));
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
+    return 0;
+}
+
+int
+loom_ctx_fill(loom_context* ctx, void* dst, const void* pattern, size_t elem_bytes, uint64_t count)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    if(!count) return 0;
+    if(!dst || !pattern) return fail(LOOM_E_ARG, "null pointer");
+    hipStream_t s = l.x->stream;
+    switch(elem_bytes)
+    {
+    case 1: LOOM_HIP(launch_fill<std::uint8_t>(dst, pattern, count, s)); break;
+    case 2: LOOM_HIP(launch_fill<std::uint16_t>(dst, pattern, count, s)); break;
+    case 4: LOOM_HIP(launch_fill<std::uint32_t>(dst, pattern, count, s)); break;
+    case 8: LOOM_HIP(launch_fill<std::uint64_t>(dst, pattern, count, s)); break;
+    default: return fail(LOOM_E_ARG, "element size %zu is not 1, 2, 4 or 8", elem_bytes);
+    }
+    LOOM_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
 int
 loom_fill(void* dst, const void* pattern, size_t elem_bytes, uint64_t count)
 {
-    if(int r = ensure_device()) return r;
-    if(!count) return 0;
-    if(!dst || !pattern) return fail(LOOM_E_ARG, "null pointer");
-    switch(elem_bytes)
-    {
-    case 1: LOOM_HIP(launch_fill<std::uint8_t>(dst, pattern, count)); break;
-    case 2: LOOM_HIP(launch_fill<std::uint16_t>(dst, pattern, count)); break;
-    case 4: LOOM_HIP(launch_fill<std::uint32_t>(dst, pattern, count)); break;
-    case 8: LOOM_HIP(launch_fill<std::uint64_t>(dst, pattern, count)); break;
-    default: return fail(LOOM_E_ARG, "element size %zu is not 1, 2, 4 or 8", elem_bytes);
-    }
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
-    return 0;
+    return loom_ctx_fill(nullptr, dst, pattern, elem_bytes, count);
 }
 
 int
@@ -692,152 +963,111 @@ loom_host_free(void* ptr)
 }
 
 int
+loom_ctx_copy_async(loom_context* ctx, void* dst, const void* src, size_t bytes, int kind,
+                    uint64_t* ticket)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    return copy_async_on(*l.x, dst, src, bytes, kind, ticket);
+}
+
+int
 loom_copy_async(void* dst, const void* src, size_t bytes, int kind, uint64_t* ticket)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
-    if(!ticket) return fail(LOOM_E_ARG, "null ticket");
-    if(bytes && (!dst || !src)) return fail(LOOM_E_ARG, "null pointer");
-    if(kind != LOOM_COPY_H2D && kind != LOOM_COPY_D2H) return fail(LOOM_E_ARG, "unknown copy kind %d", kind);
-    if(bytes)
-        LOOM_HIP(hipMemcpyAsync(dst, src, bytes,
-                                kind == LOOM_COPY_H2D ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
-                                g_dev.copy));
-    hipEvent_t ev{};
-    if(!g_dev.spare_events.empty())
-    {
-        ev = g_dev.spare_events.back();
-        g_dev.spare_events.pop_back();
-    }
-    else
-        LOOM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    LOOM_HIP(hipEventRecord(ev, g_dev.copy));
-    *ticket = ++g_dev.last_ticket;
-    g_dev.tickets.emplace_back(*ticket, ev);
-    return 0;
+    return loom_ctx_copy_async(nullptr, dst, src, bytes, kind, ticket);
+}
+
+int
+loom_ctx_copy_status(loom_context* ctx, uint64_t ticket, int* done)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    return copy_status_on(*l.x, ticket, done);
 }
 
 int
 loom_copy_status(uint64_t ticket, int* done)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
-    if(!done) return fail(LOOM_E_ARG, "null result");
-    while(!g_dev.tickets.empty())
+    return loom_ctx_copy_status(nullptr, ticket, done);
+}
+
+int
+loom_ctx_copy_wait(loom_context* ctx, uint64_t ticket)
+{
+    hipEvent_t    ev{};
+    loom_context* x = nullptr;
     {
-        const hipError_t q = hipEventQuery(g_dev.tickets.front().second);
-        if(q == hipErrorNotReady) break;
-        if(q != hipSuccess) return fail(static_cast<int>(q), "copy: %s", hipGetErrorString(q));
-        g_dev.spare_events.push_back(g_dev.tickets.front().second);
-        g_dev.tickets.pop_front();
+        Locked l;
+        if(int r = lock_context(ctx, l)) return r;
+        x = l.x;
+        if(ticket > x->last_ticket)
+            return fail(LOOM_E_ARG, "ticket %llu was never issued",
+                        static_cast<unsigned long long>(ticket));
+        // outstanding tickets are consecutive: the one asked for, if still
+        // outstanding, is at (ticket - oldest) in the queue
+        if(!x->tickets.empty() && x->tickets.front().first <= ticket)
+            ev = x->tickets[ticket - x->tickets.front().first].second;
     }
-    *done = g_dev.tickets.empty() || g_dev.tickets.front().first > ticket;
-    return 0;
+    // the event stays valid: it returns to the spare list only after it
+    // completed, and a spare is re-recorded only by a later copy whose ticket
+    // is larger, which this wait does not look at again
+    if(ev) LOOM_HIP(hipEventSynchronize(ev));
+    int done = 0;
+    return loom_ctx_copy_status(ctx, ticket, &done);
 }
 
 int
 loom_copy_wait(uint64_t ticket)
 {
-    hipEvent_t ev{};
-    {
-        std::lock_guard<std::mutex> lk(g_mutex);
-        if(int r = ensure_device()) return r;
-        if(ticket > g_dev.last_ticket) return fail(LOOM_E_ARG, "ticket %llu was never issued",
-                                                   static_cast<unsigned long long>(ticket));
-        // outstanding tickets are consecutive: the one asked for, if still
-        // outstanding, is at (ticket - oldest) in the queue
-        if(!g_dev.tickets.empty() && g_dev.tickets.front().first <= ticket)
-            ev = g_dev.tickets[ticket - g_dev.tickets.front().first].second;
-    }
-    if(ev) LOOM_HIP(hipEventSynchronize(ev));
-    int done = 0;
-    return loom_copy_status(ticket, &done);
+    return loom_ctx_copy_wait(nullptr, ticket);
+}
+
+int
+loom_ctx_synchronize(loom_context* ctx)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    LOOM_HIP(hipStreamSynchronize(l.x->stream));
+    return 0;
 }
 
 int
 loom_synchronize(void)
 {
-    if(int r = ensure_device()) return r;
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
-    return 0;
+    return loom_ctx_synchronize(nullptr);
+}
+
+int
+loom_ctx_run(loom_context* ctx, int kernel, int mode, const void* const* inputs,
+             void* const* outputs, uint64_t n, const loom_params* params, loom_run_stats* stats)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    return run_on(*l.x, kernel, mode, inputs, outputs, n, params, stats);
 }
 
 int
 loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, uint64_t n,
          const loom_params* params, loom_run_stats* stats)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
-    if(int r = check_kernel(kernel)) return r;
-    const KernelEntry& e = registry()[kernel];
-    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
-    if(stats) *stats = loom_run_stats{};
-    const IoArgs io = n ? make_io(e, inputs, outputs) : IoArgs{};
-    CtlArgs      c;
-    c.n         = n;
-    c.max_steps = params && params->max_steps ? params->max_steps : (1ull << 40);
-    c.thr       = params ? params->threshold : 0;
-    c.live_thr  = params ? params->live_threshold : (mode == LOOM_MODE_HYBRID ? 48 : 0);
-    c.refill    = params ? params->refill : 1;
-    c.regroup   = params ? params->regroup : 0;
-    c.err       = g_dev.err;
-    c.stats     = params && params->collect_stats ? g_dev.stats : nullptr;
-    LOOM_HIP(hipMemsetAsync(g_dev.err, 0, sizeof(std::uint32_t), g_dev.stream));
-    if(c.stats)
-        LOOM_HIP(hipMemsetAsync(g_dev.stats, 0, sizeof(std::uint64_t) * kStatWords, g_dev.stream));
-    int r = 0;
-    if(mode == LOOM_MODE_COMPACT)
-        r = run_compact(e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
-    else if(mode == LOOM_MODE_HYBRID)
-        r = run_hybrid(kernel, e, io, c, params, stats);
-    else
-        return fail(LOOM_E_ARG, "unknown mode %d", mode);
-    if(r) return r;
-    return finish_stats(stats, params);
+    return loom_ctx_run(nullptr, kernel, mode, inputs, outputs, n, params, stats);
+}
+
+int
+loom_ctx_run_host(loom_context* ctx, int kernel, int mode, const void* const* inputs,
+                  void* const* outputs, uint64_t n, const loom_params* params,
+                  loom_run_stats* stats)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    return run_host_on(*l.x, kernel, mode, inputs, outputs, n, params, stats);
 }
 
 int
 loom_run_host(int kernel, int mode, const void* const* inputs, void* const* outputs, uint64_t n,
               const loom_params* params, loom_run_stats* stats)
 {
-    if(int r = ensure_device()) return r;
-    if(int r = check_kernel(kernel)) return r;
-    const KernelEntry& e  = registry()[kernel];
-    // validate before allocating or copying anything
-    if(int r = validate_run(e, mode, inputs, outputs, n, params)) return r;
-    const int          ni = e.info.n_inputs, no = e.info.n_outputs;
-    std::vector<void*> din(ni), dout(no);
-    {
-        std::lock_guard<std::mutex> lk(g_mutex);
-        auto& hc = g_hostcols;
-        if(hc.ptr.size() < static_cast<std::size_t>(ni + no))
-        {
-            hc.ptr.resize(ni + no, nullptr);
-            hc.bytes.resize(ni + no, 0);
-        }
-        for(int k = 0; k < ni + no; ++k)
-        {
-            const int         t     = k < ni ? e.info.input_type[k] : e.info.output_type[k - ni];
-            const std::size_t bytes = std::max<std::size_t>(1, n * type_size(t));
-            if(hc.bytes[k] < bytes)
-            {
-                if(hc.ptr[k]) LOOM_HIP(hipFree(hc.ptr[k]));
-                LOOM_HIP(hipMalloc(&hc.ptr[k], bytes));
-                hc.bytes[k] = bytes;
-            }
-            (k < ni ? din[k] : dout[k - ni]) = hc.ptr[k];
-        }
-        for(int k = 0; k < ni && n; ++k)
-            LOOM_HIP(hipMemcpyAsync(din[k], inputs[k], n * type_size(e.info.input_type[k]),
-                                    hipMemcpyHostToDevice, g_dev.stream));
-    }
-    if(int r = loom_run(kernel, mode, din.data(), dout.data(), n, params, stats)) return r;
-    std::lock_guard<std::mutex> lk(g_mutex);
-    for(int k = 0; k < no && n; ++k)
-        LOOM_HIP(hipMemcpyAsync(outputs[k], dout[k], n * type_size(e.info.output_type[k]),
-                                hipMemcpyDeviceToHost, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
-    return 0;
+    return loom_ctx_run_host(nullptr, kernel, mode, inputs, outputs, n, params, stats);
 }
 
 int
@@ -875,47 +1105,19 @@ loom_batch_set_members(int set, int* kernels, int max)
 }
 
 int
+loom_ctx_batch_run(loom_context* ctx, int set, const loom_job* jobs, int njobs,
+                   const loom_params* params, loom_run_stats* stats)
+{
+    Locked l;
+    if(int r = lock_context(ctx, l)) return r;
+    return batch_run_on(*l.x, set, jobs, njobs, params, stats);
+}
+
+int
 loom_batch_run(int set, const loom_job* jobs, int njobs, const loom_params* params,
                loom_run_stats* stats)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
-    if(int r = validate_params(params)) return r;
-    std::vector<JobDesc> jd;
-    if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
-    if(stats) *stats = loom_run_stats{};
-    if(!g_dev.jobs) LOOM_HIP(hipMalloc(&g_dev.jobs, sizeof(JobDesc) * kMaxJobs));
-    if(!g_dev.bctl) LOOM_HIP(hipMalloc(&g_dev.bctl, sizeof(BatchCtl)));
-    LOOM_HIP(hipMemcpyAsync(g_dev.jobs, jd.data(), sizeof(JobDesc) * jd.size(),
-                            hipMemcpyHostToDevice, g_dev.stream));
-    std::uint64_t threads = 0;
-    for(const auto& j: jd) threads += j.end - j.begin;
-    const BatchEntry& b = batch_registry()[set];
-    const CtlArgs     c = make_ctl(params, threads);
-    BatchArgs         a;
-    a.jobs  = g_dev.jobs;
-    a.ctl   = g_dev.bctl;
-    a.njobs = static_cast<std::uint32_t>(njobs);
-    const unsigned grid = batch_grid(b, params, threads);
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
-                       g_dev.stats, static_cast<std::uint32_t>(kStatWords));
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
-    LOOM_HIP(hipGetLastError());
-    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    LOOM_HIP(b.launch[batch_policy(params)](c, a, grid, g_dev.stream));
-    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
-    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
-    float ms = 0;
-    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
-    if(stats)
-    {
-        stats->ms_total      = ms;
-        stats->ms_main       = ms;
-        stats->main_launches = 1;
-        stats->passes        = 1;
-        stats->grid_waves    = grid;
-    }
-    return finish_stats(stats, params);
+    return loom_ctx_batch_run(nullptr, set, jobs, njobs, params, stats);
 }
 
 } // extern "C"
@@ -924,11 +1126,20 @@ struct loom_graph
 {
     hipGraph_t     graph{};
     hipGraphExec_t exec{};
+    loom_context*  ctx  = nullptr; ///< the context whose stream and counters it uses
     JobDesc*       jobs = nullptr;
     BatchCtl*      ctl  = nullptr;
     loom_params    params{};
     bool           has_params = false;
     unsigned       grid       = 0;
+
+    ~loom_graph()
+    {
+        if(exec) (void)hipGraphExecDestroy(exec);
+        if(graph) (void)hipGraphDestroy(graph);
+        if(jobs) (void)hipFree(jobs);
+        if(ctl) (void)hipFree(ctl);
+    }
 };
 
 extern "C" {
@@ -937,13 +1148,16 @@ int
 loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_params* params,
                         loom_graph** graph)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    loom_context& x = *l.x;
     if(!graph) return fail(LOOM_E_ARG, "null graph");
     if(int r = validate_params(params)) return r;
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
-    auto g = new loom_graph;
+    // every early return below frees what was made so far (the destructor)
+    auto g = std::make_unique<loom_graph>();
+    g->ctx = &x;
     if(params)
     {
         g->params     = *params;
@@ -951,49 +1165,47 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     }
     LOOM_HIP(hipMalloc(&g->jobs, sizeof(JobDesc) * kMaxJobs));
     LOOM_HIP(hipMalloc(&g->ctl, sizeof(BatchCtl)));
-    LOOM_HIP(hipMemcpyAsync(g->jobs, jd.data(), sizeof(JobDesc) * jd.size(),
-                            hipMemcpyHostToDevice, g_dev.stream));
-    LOOM_HIP(hipStreamSynchronize(g_dev.stream));
+    LOOM_HIP(hipMemcpyAsync(g->jobs, jd.data(), sizeof(JobDesc) * jd.size(), hipMemcpyHostToDevice,
+                            x.stream));
+    LOOM_HIP(hipStreamSynchronize(x.stream));
     std::uint64_t threads = 0;
     for(const auto& j: jd) threads += j.end - j.begin;
     const BatchEntry& b = batch_registry()[set];
-    const CtlArgs     c = make_ctl(params, threads);
+    const CtlArgs     c = make_ctl(params, threads, LOOM_MODE_HYBRID, x.err, x.stats);
     BatchArgs         a;
     a.jobs  = g->jobs;
     a.ctl   = g->ctl;
     a.njobs = static_cast<std::uint32_t>(njobs);
     g->grid = batch_grid(b, params, threads);
     // counters are cleared by a kernel node (no memset nodes in the graph)
-    LOOM_HIP(hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, g_dev.stream, g_dev.err,
-                       g_dev.stats, static_cast<std::uint32_t>(kStatWords));
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, g_dev.stream, a);
-    const hipError_t le = b.launch[batch_policy(params)](c, a, g->grid, g_dev.stream);
-    const hipError_t ce = hipStreamEndCapture(g_dev.stream, &g->graph);
+    LOOM_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, x.stream, x.err, x.stats,
+                       static_cast<std::uint32_t>(kStatWords));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kMaxJobs), 0, x.stream, a);
+    const hipError_t le = b.launch[batch_policy(params)](c, a, g->grid, x.stream);
+    const hipError_t ce = hipStreamEndCapture(x.stream, &g->graph);
     if(le != hipSuccess || ce != hipSuccess)
-    {
-        delete g;
         return fail(LOOM_E_RUNTIME, "graph capture failed: %s",
                     hipGetErrorString(le != hipSuccess ? le : ce));
-    }
     LOOM_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
-    *graph = g;
+    *graph = g.release();
     return 0;
 }
 
 int
 loom_graph_launch(loom_graph* g, loom_run_stats* stats)
 {
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(int r = ensure_device()) return r;
     if(!g) return fail(LOOM_E_ARG, "null graph");
+    Locked l;
+    if(int r = lock_context(g->ctx, l)) return r;
+    loom_context& x = *l.x;
     if(stats) *stats = loom_run_stats{};
-    LOOM_HIP(hipEventRecord(g_dev.ev[0], g_dev.stream));
-    LOOM_HIP(hipGraphLaunch(g->exec, g_dev.stream));
-    LOOM_HIP(hipEventRecord(g_dev.ev[1], g_dev.stream));
-    LOOM_HIP(hipEventSynchronize(g_dev.ev[1]));
+    LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
+    LOOM_HIP(hipGraphLaunch(g->exec, x.stream));
+    LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
+    LOOM_HIP(hipEventSynchronize(x.ev[1]));
     float ms = 0;
-    LOOM_HIP(hipEventElapsedTime(&ms, g_dev.ev[0], g_dev.ev[1]));
+    LOOM_HIP(hipEventElapsedTime(&ms, x.ev[0], x.ev[1]));
     if(stats)
     {
         stats->ms_total      = ms;
@@ -1002,18 +1214,17 @@ loom_graph_launch(loom_graph* g, loom_run_stats* stats)
         stats->passes        = 1;
         stats->grid_waves    = g->grid;
     }
-    return finish_stats(stats, g->has_params ? &g->params : nullptr);
+    return finish_stats(x, stats, g->has_params ? &g->params : nullptr);
 }
 
 int
 loom_graph_destroy(loom_graph* g)
 {
     if(!g) return 0;
-    std::lock_guard<std::mutex> lk(g_mutex);
-    if(g->exec) hipGraphExecDestroy(g->exec);
-    if(g->graph) hipGraphDestroy(g->graph);
-    if(g->jobs) hipFree(g->jobs);
-    if(g->ctl) hipFree(g->ctl);
+    {
+        Locked l;
+        if(lock_context(g->ctx, l) == 0) (void)hipStreamSynchronize(l.x->stream);
+    }
     delete g;
     return 0;
 }

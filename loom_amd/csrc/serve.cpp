@@ -2,8 +2,8 @@
 /// @file serve.cpp
 /// @brief Continuous batching of kernel launches over the batch launch
 /// (loom_server_* in loom_amd.h).  Host code only, written against the
-/// public C ABI of the library: device memory, copies, fills and
-/// loom_batch_run.
+/// public C ABI of the library: device memory, and copies, fills and batch
+/// launches on an execution context of the server's own.
 ///
 /// The nearest relative in the reference is the Shop's exchange loop: a
 /// stream admits new inputs as old threads halt (src/vm/mp.hpp:61-131,
@@ -11,14 +11,23 @@
 /// whole request (a launch of n threads): it becomes resident when its
 /// device columns fit the HBM budget, and every step advances ALL resident
 /// requests by a slice of their threads in ONE persistent batch launch.
+///
+/// Requests come from a source: a closed trace with arrival times
+/// (loom_server_run), or submissions from any host thread while the loop
+/// runs on a thread of its own (loom_server_start / submit / poll / stop:
+/// the Shop's main thread exchanging while its workers compute,
+/// src/vm/mp.hpp:349-369).
 #include <loom_amd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -109,7 +118,7 @@ class Arena
 
 struct Resident
 {
-    std::uint64_t      req    = 0;
+    loom_request*      req    = nullptr;
     std::uint64_t      cursor = 0;
     std::uint64_t      off    = 0, bytes = 0;
     std::uint64_t      ticket = 0; ///< last copy of this request (0: none)
@@ -117,66 +126,113 @@ struct Resident
 };
 } // namespace
 
+
+namespace
+{
+using clock_type = std::chrono::steady_clock;
+
+/// @brief Where requests come from.  `take` moves the requests that have
+/// arrived by `now` (ms since the loop started) into `out` and returns
+/// false once no request will ever come again; `idle` blocks until one may
+/// arrive (at most `max_ms`).
+struct Source
+{
+    virtual ~Source()                                                  = default;
+    virtual bool take(double now, std::vector<loom_request*>& out)     = 0;
+    virtual void idle(double now, double max_ms)                       = 0;
+};
+
+/// A closed trace sorted by arrival time, replayed against the clock
+/// (realtime) or all arrived at the start.
+struct TraceSource final : Source
+{
+    loom_request* reqs;
+    std::uint64_t count, next = 0;
+    bool          realtime;
+
+    TraceSource(loom_request* r, std::uint64_t c, bool rt) : reqs(r), count(c), realtime(rt) {}
+
+    bool
+    take(double now, std::vector<loom_request*>& out) override
+    {
+        while(next < count && (!realtime || reqs[next].arrival_ms <= now)) out.push_back(&reqs[next++]);
+        return next < count;
+    }
+
+    void
+    idle(double now, double max_ms) override
+    {
+        if(next >= count || !realtime) return;
+        const double wait = std::min(reqs[next].arrival_ms - now, max_ms);
+        if(wait > 0.05)
+            std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait - 0.02));
+    }
+};
+
+/// Submissions from other host threads (a locked queue the loop drains).
+struct OnlineSource final : Source
+{
+    std::mutex                mu;
+    std::condition_variable   cv;
+    std::deque<loom_request*> queue;
+    bool                      open     = false; ///< submissions accepted
+    bool                      stopping = false;
+
+    bool
+    take(double, std::vector<loom_request*>& out) override
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        while(!queue.empty())
+        {
+            out.push_back(queue.front());
+            queue.pop_front();
+        }
+        return !stopping;
+    }
+
+    void
+    idle(double, double max_ms) override
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait_for(lk, std::chrono::duration<double, std::milli>(max_ms),
+                    [&] { return !queue.empty() || stopping; });
+    }
+};
+} // namespace
+
 struct loom_server
 {
     loom_server_config cfg{};
+    loom_context*      ctx  = nullptr; // the server's own kernel and copy streams
     char*              base = nullptr; // the arena's device block
     Arena              arena;
     std::vector<int>   members;        // global kernel indices of the set
+    // online serving
+    std::mutex             mu;          // start / stop
+    OnlineSource           online;
+    std::thread            loop;
+    bool                   running = false;
+    clock_type::time_point t0{};
+    int                    loop_status = 0;
+    loom_server_stats      loop_stats{};
 };
 
-extern "C" {
-
-int
-loom_server_create(const loom_server_config* config, loom_server** server)
+namespace
 {
-    if(!config || !server) return LOOM_E_ARG;
-    if(config->max_resident < 1 || config->max_resident > LOOM_MAX_JOBS) return LOOM_E_ARG;
-    if(config->hbm_budget < kAlign || config->step_threads < 64) return LOOM_E_ARG;
-    auto s = new loom_server;
-    s->cfg = *config;
-    int buf[LOOM_MAX_JOBS];
-    const int nm = loom_batch_set_members(config->batch_set, buf, LOOM_MAX_JOBS);
-    if(nm <= 0)
-    {
-        delete s;
-        return nm < 0 ? -nm : LOOM_E_KERNEL;
-    }
-    s->members.assign(buf, buf + std::min(nm, static_cast<int>(LOOM_MAX_JOBS)));
-    void* p = nullptr;
-    if(int r = loom_malloc(&p, config->hbm_budget))
-    {
-        delete s;
-        return r;
-    }
-    s->base = static_cast<char*>(p);
-    s->arena.reset(config->hbm_budget);
-    *server = s;
-    return 0;
+void
+mark_complete(loom_request& q)
+{
+    __atomic_store_n(&q.complete, 1, __ATOMIC_RELEASE);
 }
 
+/// @brief The serving loop over requests from @p src until it is exhausted
+/// and every request it gave has completed.
 int
-loom_server_destroy(loom_server* server)
+serve(loom_server* s, Source& src, clock_type::time_point t0, loom_server_stats* stats)
 {
-    if(!server) return 0;
-    int r = 0;
-    if(server->base) r = loom_free(server->base);
-    delete server;
-    return r;
-}
-
-int
-loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime,
-                loom_server_stats* stats)
-{
-    using clock = std::chrono::steady_clock;
-    if(!s || (!reqs && count)) return LOOM_E_ARG;
-    for(std::uint64_t i = 1; i < count; ++i)
-        if(reqs[i].arrival_ms < reqs[i - 1].arrival_ms) return LOOM_E_ARG;
-
+    loom_context*     ctx = s->ctx;
     loom_server_stats st{};
-    const auto        t0  = clock::now();
-    auto              now = [&] { return std::chrono::duration<double, std::milli>(clock::now() - t0).count(); };
+    auto now = [&] { return std::chrono::duration<double, std::milli>(clock_type::now() - t0).count(); };
 
     // kernel infos of the members
     std::map<int, loom_kernel_info> info;
@@ -186,23 +242,16 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         if(int r = loom_kernel_info_get(k, &ki)) return r;
         info.emplace(k, ki);
     }
-    for(std::uint64_t i = 0; i < count; ++i)
-    {
-        reqs[i].status   = 0;
-        reqs[i].steps    = 0;
-        reqs[i].admit_ms = reqs[i].done_ms = 0;
-        if(!info.count(reqs[i].kernel)) reqs[i].status = LOOM_E_KERNEL;
-        else if(reqs[i].n > 0xFFFFFFFFull) reqs[i].status = LOOM_E_CAPACITY;
-    }
 
     // A request moves through: pending (arrived, waiting for a slot and
     // memory) -> loading (inputs copying in on the copy stream) -> running
     // (in the steps) -> draining (outputs copying out) -> done.  Copies of
     // one request overlap the steps of the others.
-    std::deque<std::uint64_t> pending;
-    std::vector<Resident>     loading, running, draining;
-    std::uint64_t             next = 0, finished = 0;
-    std::uint64_t             last_ticket = 0; // latest copy issued by this run
+    std::deque<loom_request*>  pending;
+    std::vector<Resident>      loading, running, draining;
+    std::vector<loom_request*> arrived;
+    std::uint64_t              last_ticket = 0; // latest copy issued by this loop
+    bool                       more        = true;
 
     auto request_bytes = [&](const loom_request& q, std::vector<std::uint64_t>& cols)
     {
@@ -220,16 +269,24 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         return b;
     };
 
+    auto finish = [&](loom_request& q)
+    {
+        q.done_ms = now();
+        if(q.status == 0) st.threads += q.n;
+        mark_complete(q);
+    };
+
     // start copying the outputs of a request whose threads all ran
     auto drain = [&](Resident& r) -> int
     {
-        loom_request&           q  = reqs[r.req];
+        loom_request&           q  = *r.req;
         const loom_kernel_info& ki = info.at(q.kernel);
         r.ticket                   = 0;
         for(int k = 0; k < ki.n_outputs && q.n; ++k)
         {
-            if(int e = loom_copy_async(q.outputs[k], r.out[k], q.n * type_bytes(ki.output_type[k]),
-                                       LOOM_COPY_D2H, &r.ticket))
+            if(int e = loom_ctx_copy_async(ctx, q.outputs[k], r.out[k],
+                                           q.n * type_bytes(ki.output_type[k]), LOOM_COPY_D2H,
+                                           &r.ticket))
                 return e;
             last_ticket = r.ticket;
         }
@@ -240,22 +297,62 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
     auto ready = [&](const Resident& r, int& done) -> int
     {
         done = 1;
-        return r.ticket ? loom_copy_status(r.ticket, &done) : 0;
+        return r.ticket ? loom_ctx_copy_status(ctx, r.ticket, &done) : 0;
     };
 
     // on an error, let the copies already issued land before the arena is
-    // handed to another run, then give the whole arena back
+    // reused, give the whole arena back, and complete every request the
+    // loop holds with the error
     auto fail = [&](int e) -> int
     {
-        if(last_ticket) loom_copy_wait(last_ticket);
-        loom_synchronize();
+        if(last_ticket) loom_ctx_copy_wait(ctx, last_ticket);
+        loom_ctx_synchronize(ctx);
         s->arena.reset(s->cfg.hbm_budget);
+        for(auto* v: {&loading, &running, &draining})
+            for(auto& r: *v)
+            {
+                if(!r.req->status) r.req->status = e;
+                finish(*r.req);
+            }
+        for(auto* q: pending)
+        {
+            q->status = e;
+            finish(*q);
+        }
+        // requests still to come are refused with the same error
+        for(;;)
+        {
+            arrived.clear();
+            const bool again = src.take(1e300, arrived);
+            for(auto* q: arrived)
+            {
+                q->status = e;
+                finish(*q);
+            }
+            if(!again) break;
+            src.idle(now(), 1.0);
+        }
         return e;
+    };
+
+    auto admit_new = [&](loom_request* q)
+    {
+        q->status   = 0;
+        q->steps    = 0;
+        q->admit_ms = q->done_ms = 0;
+        if(!info.count(q->kernel))
+            q->status = LOOM_E_KERNEL;
+        else if(q->n > 0xFFFFFFFFull)
+            q->status = LOOM_E_CAPACITY;
+        if(q->status)
+            finish(*q); // rejected before admission
+        else
+            pending.push_back(q);
     };
 
     std::vector<std::uint64_t> cols;
     std::vector<loom_job>      jobs;
-    while(finished < count)
+    for(;;)
     {
         // ---- retire requests whose outputs arrived on the host ----
         for(std::size_t i = 0; i < draining.size();)
@@ -267,33 +364,26 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
                 ++i;
                 continue;
             }
-            loom_request& q = reqs[draining[i].req];
             s->arena.release(draining[i].off, draining[i].bytes);
-            q.done_ms = now();
-            st.threads += q.n;
-            ++finished;
+            finish(*draining[i].req);
             draining.erase(draining.begin() + static_cast<std::ptrdiff_t>(i));
         }
         // ---- arrivals ----
-        const double t = now();
-        while(next < count && (!realtime || reqs[next].arrival_ms <= t))
-        {
-            if(reqs[next].status)
-                ++finished; // rejected before admission
-            else
-                pending.push_back(next);
-            ++next;
-        }
+        arrived.clear();
+        more = src.take(now(), arrived);
+        for(auto* q: arrived) admit_new(q);
+        if(!more && pending.empty() && loading.empty() && running.empty() && draining.empty())
+            break;
         // ---- admission: FIFO, by slot and HBM budget; inputs start copying ----
         while(!pending.empty() && loading.size() + running.size() < s->cfg.max_resident)
         {
-            loom_request&       q     = reqs[pending.front()];
+            loom_request&       q     = *pending.front();
             const std::uint64_t bytes = request_bytes(q, cols);
             if(bytes > s->cfg.hbm_budget)
             {
                 q.status = LOOM_E_CAPACITY;
                 pending.pop_front();
-                ++finished;
+                finish(q);
                 continue;
             }
             std::uint64_t off = 0;
@@ -313,10 +403,15 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             {
                 const std::size_t tb = type_bytes(ki.input_type[k]);
                 const int         e  = (q.broadcast >> k) & 1u ?
-                                           loom_fill(r.in[k], &q.scalars[k], tb, q.n) :
-                                           loom_copy_async(r.in[k], q.inputs[k], q.n * tb,
-                                                           LOOM_COPY_H2D, &r.ticket);
-                if(e) return fail(e);
+                                           loom_ctx_fill(ctx, r.in[k], &q.scalars[k], tb, q.n) :
+                                           loom_ctx_copy_async(ctx, r.in[k], q.inputs[k], q.n * tb,
+                                                               LOOM_COPY_H2D, &r.ticket);
+                if(e)
+                {
+                    running.push_back(std::move(r)); // completed with the error by fail
+                    pending.pop_front();
+                    return fail(e);
+                }
                 if(r.ticket) last_ticket = r.ticket;
             }
             q.admit_ms = now();
@@ -349,18 +444,14 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             // nothing to run: wait for a copy, or idle until the next arrival
             if(!loading.empty())
             {
-                if(int e = loom_copy_wait(loading.front().ticket)) return fail(e);
+                if(int e = loom_ctx_copy_wait(ctx, loading.front().ticket)) return fail(e);
             }
             else if(!draining.empty())
             {
-                if(int e = loom_copy_wait(draining.front().ticket)) return fail(e);
+                if(int e = loom_ctx_copy_wait(ctx, draining.front().ticket)) return fail(e);
             }
-            else if(next < count && realtime)
-            {
-                const double wait = reqs[next].arrival_ms - now();
-                if(wait > 0.05)
-                    std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait - 0.02));
-            }
+            else if(more)
+                src.idle(now(), 50.0);
             continue;
         }
         // ---- one step: a slice of every running request in one batch launch ----
@@ -369,7 +460,7 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         jobs.assign(running.size(), loom_job{});
         for(std::size_t i = 0; i < running.size(); ++i)
         {
-            const loom_request& q = reqs[running[i].req];
+            const loom_request& q = *running[i].req;
             loom_job&           j = jobs[i];
             j.kernel              = q.kernel;
             j.begin               = running[i].cursor;
@@ -378,10 +469,10 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
             for(std::size_t k = 0; k < running[i].out.size(); ++k) j.outputs[k] = running[i].out[k];
         }
         loom_run_stats rs{};
-        if(int e = loom_batch_run(s->cfg.batch_set, jobs.data(), static_cast<int>(jobs.size()),
-                                  &s->cfg.params, &rs))
+        if(int e = loom_ctx_batch_run(ctx, s->cfg.batch_set, jobs.data(),
+                                      static_cast<int>(jobs.size()), &s->cfg.params, &rs))
         {
-            for(auto& r: running) reqs[r.req].status = e;
+            for(auto& r: running) r.req->status = e;
             return fail(e);
         }
         st.gpu_ms += rs.ms_main;
@@ -391,8 +482,8 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
         for(std::size_t i = 0; i < running.size(); ++i)
         {
             running[i].cursor = jobs[i].end;
-            ++reqs[running[i].req].steps;
-            if(running[i].cursor >= reqs[running[i].req].n)
+            ++running[i].req->steps;
+            if(running[i].cursor >= running[i].req->n)
             {
                 if(int e = drain(running[i])) return fail(e);
             }
@@ -404,6 +495,136 @@ loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime
     st.makespan_ms = now();
     if(stats) *stats = st;
     return 0;
+}
+} // namespace
+
+extern "C" {
+
+int
+loom_server_create(const loom_server_config* config, loom_server** server)
+{
+    if(!config || !server) return LOOM_E_ARG;
+    if(config->max_resident < 1 || config->max_resident > LOOM_MAX_JOBS) return LOOM_E_ARG;
+    if(config->hbm_budget < kAlign || config->step_threads < 64) return LOOM_E_ARG;
+    auto s = std::make_unique<loom_server>();
+    s->cfg = *config;
+    int buf[LOOM_MAX_JOBS];
+    const int nm = loom_batch_set_members(config->batch_set, buf, LOOM_MAX_JOBS);
+    if(nm <= 0) return nm < 0 ? -nm : LOOM_E_KERNEL;
+    s->members.assign(buf, buf + std::min(nm, static_cast<int>(LOOM_MAX_JOBS)));
+    if(int r = loom_context_create(&s->ctx)) return r;
+    void* p = nullptr;
+    if(int r = loom_malloc(&p, config->hbm_budget))
+    {
+        loom_context_destroy(s->ctx);
+        return r;
+    }
+    s->base = static_cast<char*>(p);
+    s->arena.reset(config->hbm_budget);
+    *server = s.release();
+    return 0;
+}
+
+int
+loom_server_destroy(loom_server* server)
+{
+    if(!server) return 0;
+    loom_server_stop(server, nullptr);
+    int r = 0;
+    if(server->ctx) loom_context_destroy(server->ctx);
+    if(server->base) r = loom_free(server->base);
+    delete server;
+    return r;
+}
+
+int
+loom_server_run(loom_server* s, loom_request* reqs, uint64_t count, int realtime,
+                loom_server_stats* stats)
+{
+    if(!s || (!reqs && count)) return LOOM_E_ARG;
+    for(std::uint64_t i = 1; i < count; ++i)
+        if(reqs[i].arrival_ms < reqs[i - 1].arrival_ms) return LOOM_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if(s->running) return LOOM_E_STATE;
+        s->running = true;
+    }
+    for(std::uint64_t i = 0; i < count; ++i) reqs[i].complete = 0;
+    TraceSource src(reqs, count, realtime != 0);
+    const int   r = serve(s, src, clock_type::now(), stats);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->running = false;
+    return r;
+}
+
+int
+loom_server_start(loom_server* s)
+{
+    if(!s) return LOOM_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if(s->running) return LOOM_E_STATE;
+    {
+        std::lock_guard<std::mutex> q(s->online.mu);
+        s->online.stopping = false;
+        s->online.queue.clear();
+    }
+    s->running     = true;
+    s->loop_status = 0;
+    s->loop_stats  = loom_server_stats{};
+    s->t0          = clock_type::now();
+    s->loop = std::thread([s] { s->loop_status = serve(s, s->online, s->t0, &s->loop_stats); });
+    std::lock_guard<std::mutex> q(s->online.mu);
+    s->online.open = true;
+    return 0;
+}
+
+int
+loom_server_submit(loom_server* s, loom_request* q)
+{
+    if(!s || !q) return LOOM_E_ARG;
+    std::lock_guard<std::mutex> lk(s->online.mu);
+    if(!s->online.open || s->online.stopping) return LOOM_E_STATE;
+    q->complete   = 0;
+    q->status     = 0;
+    q->arrival_ms = std::chrono::duration<double, std::milli>(clock_type::now() - s->t0).count();
+    s->online.queue.push_back(q);
+    s->online.cv.notify_one();
+    return 0;
+}
+
+int
+loom_server_poll(loom_server* s, const loom_request* q, int* done)
+{
+    if(!s || !q || !done) return LOOM_E_ARG;
+    *done = __atomic_load_n(&q->complete, __ATOMIC_ACQUIRE) != 0;
+    return 0;
+}
+
+int
+loom_server_wait(loom_server* s, const loom_request* q)
+{
+    if(!s || !q) return LOOM_E_ARG;
+    while(!__atomic_load_n(&q->complete, __ATOMIC_ACQUIRE))
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    return q->status;
+}
+
+int
+loom_server_stop(loom_server* s, loom_server_stats* stats)
+{
+    if(!s) return LOOM_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if(!s->loop.joinable()) return 0;
+    {
+        std::lock_guard<std::mutex> q(s->online.mu);
+        s->online.open     = false;
+        s->online.stopping = true;
+        s->online.cv.notify_one();
+    }
+    s->loop.join();
+    s->running = false;
+    if(stats) *stats = s->loop_stats;
+    return s->loop_status;
 }
 
 } // extern "C"

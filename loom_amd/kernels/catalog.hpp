@@ -19,6 +19,8 @@
     X("int_mix_u32", loom_amd::programs::int_mix<std::uint32_t>)      \
     X("fp_mix_f32", loom_amd::programs::fp_mix<float>)                \
     X("fp_mix_f64", loom_amd::programs::fp_mix<double>)               \
+    X("sincos_f32", loom_amd::programs::sincos<float>)                \
+    X("sincos_f64", loom_amd::programs::sincos<double>)               \
     X("threefry2x32_u32", loom_amd::programs::threefry2x32<std::uint32_t>) \
     X("cubic_bisect_f32", loom_amd::programs::cubic_bisect<float>)    \
     X("cubic_bisect_f64", loom_amd::programs::cubic_bisect<double>)

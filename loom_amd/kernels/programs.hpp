@@ -145,6 +145,15 @@ TKERNEL2(int_mix, a, b)
     IADD(la, la, p2)
 TKERNEL_END6(s, d, m, q, x1, la);
 
+// Sine and cosine of x: the op contract's polynomials (the reference SIMD
+// backends', simd.hpp:955-1100), valid on [-pi, pi].  Checked bit for bit
+// against vectors evaluated in exact arithmetic from the reference's tables.
+template <std::floating_point T>
+TKERNEL1(sincos, x)
+    TSIN(s, x),
+    TCOS(c, x)
+TKERNEL_END2(s, c);
+
 // Float op mix, straight-line, exercising the floating-point contract:
 // arithmetic, fused multiply-add/sub, sqrt/rsqrt, rounding, min/max,
 // copysign, sin/cos polynomials, select, union register and conversions.

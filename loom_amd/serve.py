@@ -171,6 +171,36 @@ class Server:
                                              C.byref(st)), "loom_server_run")
         return reqs, st, outs
 
+    # ---- online serving: the loop on a thread of its own, submissions from
+    # any thread (loom_server_start / submit / wait / stop) ----
+    def start(self):
+        abi.check(abi.load().loom_server_start(self.handle), "loom_server_start")
+
+    def submit(self, spec: RequestSpec, pools: Pools, outputs=None) -> "Ticket":
+        """Submit one request while the loop runs (thread-safe); its arrival
+        is now.  The returned ticket keeps the request and its columns alive
+        until it is complete."""
+        k = kernel(spec.kernel)
+        q = abi.Request()
+        q.kernel = k.index
+        q.n = spec.n
+        ins = pools.inputs_of(spec)
+        q.inputs[0] = ins[0].ctypes.data
+        if spec.kernel.startswith("smallpt9"):
+            q.broadcast = 1 << 1
+            q.scalars[1] = workloads.pt_job(spec.w, spec.h, spec.spp)
+        outs = outputs if outputs is not None else [np.zeros(spec.n, dt) for dt in k.outputs]
+        for j, o in enumerate(outs):
+            q.outputs[j] = o.ctypes.data
+        t = Ticket(q, ins, outs, self)
+        abi.check(abi.load().loom_server_submit(self.handle, C.byref(q)), "loom_server_submit")
+        return t
+
+    def stop(self):
+        st = abi.ServerStats()
+        abi.check(abi.load().loom_server_stop(self.handle, C.byref(st)), "loom_server_stop")
+        return st
+
     def close(self):
         if self.handle:
             abi.load().loom_server_destroy(self.handle)
@@ -181,6 +211,26 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class Ticket:
+    """A submitted request: the C struct the server fills, and the host
+    columns it reads and writes."""
+    request: abi.Request
+    inputs: list
+    outputs: list
+    server: Server
+
+    def done(self) -> bool:
+        d = C.c_int()
+        abi.check(abi.load().loom_server_poll(self.server.handle, C.byref(self.request),
+                                              C.byref(d)), "loom_server_poll")
+        return bool(d.value)
+
+    def wait(self) -> int:
+        """Block until complete; returns the request's status."""
+        return abi.load().loom_server_wait(self.server.handle, C.byref(self.request))
 
 
 def latency_summary(reqs, st) -> dict:

@@ -14,7 +14,7 @@ front-end, from the reference checkout where it lies, into oracle/_ref/:
 Only tests load these.  Nothing is built when the reference is absent (the
 GPU box has none; the libraries built here travel with the tree).
 
-    python oracle/build_ref.py [--ref /root/reference]
+    python oracle/build_ref.py [--ref /root/reference] [--oracle-only]
 """
 from __future__ import annotations
 
@@ -41,7 +41,7 @@ def _run(cmd):
         raise RuntimeError(f"{' '.join(cmd[:3])} ... failed\n{r.stderr[-6000:]}")
 
 
-def build(ref: Path, before_link=None) -> list[Path]:
+def build(ref: Path, before_link=None, oracle_only: bool = False) -> list[Path]:
     """Build both libraries; `before_link` (if given) is called after the
     objects are compiled and before the kernel library is linked against
     libloom_amd.so, so a caller can compile this concurrently with that
@@ -71,6 +71,8 @@ def build(ref: Path, before_link=None) -> list[Path]:
               f"-I{ext}", "-x", "hip", "-c", str(HERE / name), "-o", str(o)])
         return o
 
+    if oracle_only:
+        return [oracle()]
     with cf.ThreadPoolExecutor(min(len(SOURCES) + 1, 8)) as ex:
         fo = ex.submit(oracle)
         objs = list(ex.map(obj, SOURCES))
@@ -100,8 +102,10 @@ def build(ref: Path, before_link=None) -> list[Path]:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", type=Path, default=Path("/root/reference"))
+    ap.add_argument("--oracle-only", action="store_true",
+                    help="only the oracle interpreter over the reference's kernels")
     a = ap.parse_args(argv)
-    for p in build(a.ref):
+    for p in build(a.ref, oracle_only=a.oracle_only):
         print("built", p, flush=True)
     return 0
 

@@ -285,6 +285,29 @@ oracle_register_name(int k, int r, char* buf, size_t len)
     return 0;
 }
 
+/// Literal operand i of kernel k, in statement order: its text (NUL-terminated,
+/// at most len bytes), type code (loom_type) and the bit pattern the kernel
+/// computes with.  Returns 1 past the last literal.
+int
+oracle_kernel_literal(int k, int i, char* text, size_t len, int* type, uint64_t* bits)
+{
+    if(k < 0 || k >= g_count || i < 0 || !text || !len) return 1;
+    const Program& p = programs()[k];
+    int            n = 0;
+    for(const auto& s: p.code)
+        for(const auto& a: s.args)
+        {
+            if(!a.literal || n++ != i) continue;
+            const std::size_t m = std::min(len - 1, a.name.size());
+            std::memcpy(text, a.name.data(), m);
+            text[m] = 0;
+            *type   = static_cast<int>(a.ty);
+            *bits   = a.bits;
+            return 0;
+        }
+    return 1;
+}
+
 int
 oracle_parse_f32(const char* s, float* v)
 {
@@ -348,6 +371,48 @@ oracle_smallpt9_direct(const uint32_t* sid, const uint32_t* job, float* lx, floa
         for(auto& th: pool) th.join();
     }
     return 0;
+}
+
+/// Collatz total stopping time by its definition (x -> x/2 if even, 3x+1
+/// if odd, counted until x == 1; 0 for x <= 1), written directly in C++ and
+/// independent of the IR: the brute force the Collatz kernel is checked
+/// against.  Returns 1 if a trajectory would leave 64 bits.
+int
+oracle_collatz_direct(const uint64_t* x, uint32_t* steps, uint64_t count, int threads)
+{
+    std::atomic<int> overflow{0};
+    auto work = [&](std::uint64_t b, std::uint64_t e)
+    {
+        for(std::uint64_t i = b; i < e; ++i)
+        {
+            std::uint64_t v = x[i];
+            std::uint32_t n = 0;
+            while(v > 1)
+            {
+                if(v & 1)
+                {
+                    if(v > (~std::uint64_t(0) - 1) / 3) overflow = 1;
+                    v = 3 * v + 1;
+                }
+                else
+                    v >>= 1;
+                ++n;
+            }
+            steps[i] = n;
+        }
+    };
+    threads = std::max(1, threads);
+    std::vector<std::thread> pool;
+    const std::uint64_t      chunk = (count + threads - 1) / threads;
+    for(int t = 0; t < threads; ++t)
+    {
+        const std::uint64_t b = chunk * t;
+        const std::uint64_t e = std::min(count, b + chunk);
+        if(b >= e) break;
+        pool.emplace_back(work, b, e);
+    }
+    for(auto& th: pool) th.join();
+    return overflow.load();
 }
 
 float

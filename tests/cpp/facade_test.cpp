@@ -11,10 +11,12 @@
 #include <cstdint>
 #include <cstdio>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace hip = loom::vm::hip;
 using loom_amd::programs::collatz;
+using loom_amd::programs::factorial;
 using loom_amd::programs::fib;
 
 static int failures = 0;
@@ -43,6 +45,14 @@ fib_closed(std::uint64_t n)
         b                     = c;
     }
     return a;
+}
+
+static std::uint64_t
+factorial_closed(std::uint64_t a)
+{
+    std::uint64_t r = 1;
+    for(std::uint64_t k = 2; k <= a; ++k) r *= k;
+    return r;
 }
 
 static std::uint32_t
@@ -130,6 +140,102 @@ main()
         std::ranges::sort(out);
         std::ranges::sort(want);
         EXPECT(out == want, "stream outputs differ");
+    }
+    // stream bookkeeping: every row crosses PCIe once in and once out, so
+    // the bytes moved are proportional to the threads exchanged
+    {
+        std::vector<std::uint64_t> x(5000);
+        for(std::size_t i = 0; i < x.size(); ++i) x[i] = 1 + (i * 40503u) % 70000;
+        hip::params p;
+        p.live_threshold = 32;
+        hip::hybrid_stream<collatz<std::uint64_t>> s(512, p);
+        auto out = drive_stream(s, x);
+        EXPECT(out.size() == x.size(), "stream size %zu", out.size());
+        const loom_stream_counters c = s.counters();
+        EXPECT(c.rows_in == x.size() && c.rows_out == x.size(), "rows %llu in, %llu out",
+               (unsigned long long)c.rows_in, (unsigned long long)c.rows_out);
+        // 8 B of input per row in; 4 B of output + 8 B of sequence number out
+        EXPECT(c.bytes_h2d == 8 * x.size(), "bytes in %llu", (unsigned long long)c.bytes_h2d);
+        EXPECT(c.bytes_d2h == 12 * x.size(), "bytes out %llu", (unsigned long long)c.bytes_d2h);
+        EXPECT(c.exchanges > 1, "exchanges %llu", (unsigned long long)c.exchanges);
+        std::vector<std::uint32_t> want(x.size());
+        for(std::size_t i = 0; i < x.size(); ++i) want[i] = collatz_steps(x[i]);
+        std::ranges::sort(out);
+        std::ranges::sort(want);
+        EXPECT(out == want, "stream (live threshold 32) outputs differ");
+    }
+    // several host threads, each with its own execution context, running
+    // the same kernel at once: every result equals the closed form
+    {
+        std::vector<std::thread> ts;
+        std::vector<int>         bad(4, 0);
+        for(int t = 0; t < 4; ++t)
+            ts.emplace_back(
+                [t, &bad]
+                {
+                    loom_context* ctx = nullptr;
+                    if(loom_context_create(&ctx) != 0)
+                    {
+                        bad[t] = -1;
+                        return;
+                    }
+                    const int k = hip::kernel_index<collatz<std::uint64_t>>();
+                    std::vector<std::uint64_t> x(20000 + 1000 * t);
+                    for(std::size_t i = 0; i < x.size(); ++i) x[i] = 1 + (i * 7u + t) % 500000;
+                    std::vector<std::uint32_t> y(x.size());
+                    const void*                in[1]  = {x.data()};
+                    void*                      out[1] = {y.data()};
+                    loom_params                p{};
+                    p.live_threshold = 48;
+                    p.refill         = 1;
+                    p.reconv         = 2;
+                    for(int rep = 0; rep < 5; ++rep)
+                    {
+                        if(loom_ctx_run_host(ctx, k, LOOM_MODE_HYBRID, in, out, x.size(), &p,
+                                             nullptr) != 0)
+                            bad[t] = -2;
+                        for(std::size_t i = 0; i < x.size(); ++i)
+                            if(y[i] != collatz_steps(x[i])) ++bad[t];
+                    }
+                    loom_context_destroy(ctx);
+                });
+        for(auto& th: ts) th.join();
+        for(int t = 0; t < 4; ++t) EXPECT(bad[t] == 0, "context thread %d: %d", t, bad[t]);
+    }
+    // the Shop: 4 worker threads x 3 kernels, each worker a device-resident
+    // stream per kernel, the main thread exchanging (reference
+    // src/tests/vm_mp.cpp:90-150); outputs compared as sorted multisets
+    {
+        std::vector<std::uint64_t> in_fac(3000), in_fib(5000), in_cz(7000);
+        for(std::size_t i = 0; i < in_fac.size(); ++i) in_fac[i] = i % 21;
+        for(std::size_t i = 0; i < in_fib.size(); ++i) in_fib[i] = (i * 31) % 91;
+        for(std::size_t i = 0; i < in_cz.size(); ++i) in_cz[i] = 1 + (i * 2654435761u) % 300000;
+        std::vector<std::uint64_t> out_fac, out_fib;
+        std::vector<std::uint32_t> out_cz;
+        hip::params p;
+        p.live_threshold = 32;
+        hip::shop<factorial<std::uint64_t>, fib<std::uint64_t>, collatz<std::uint64_t>> shop(
+            4, 256, p);
+        auto oc = std::make_tuple(std::back_inserter(out_fac), std::back_inserter(out_fib),
+                                  std::back_inserter(out_cz));
+        auto ic = shop.bind_inputs(in_fac, in_fib, in_cz);
+        {
+            auto pool = shop.get_thread_pool();
+            while(!hip::is_complete(ic)) shop.exchange(ic, oc);
+            shop.set();
+        }
+        shop.finalize(oc);
+        std::vector<std::uint64_t> w_fac, w_fib;
+        std::vector<std::uint32_t> w_cz;
+        for(auto a: in_fac) w_fac.push_back(factorial_closed(a));
+        for(auto n: in_fib) w_fib.push_back(fib_closed(n));
+        for(auto x: in_cz) w_cz.push_back(collatz_steps(x));
+        for(auto* v: {&out_fac, &out_fib, &w_fac, &w_fib}) std::ranges::sort(*v);
+        std::ranges::sort(out_cz);
+        std::ranges::sort(w_cz);
+        EXPECT(out_fac == w_fac, "shop factorial outputs differ (%zu)", out_fac.size());
+        EXPECT(out_fib == w_fib, "shop fib outputs differ (%zu)", out_fib.size());
+        EXPECT(out_cz == w_cz, "shop collatz outputs differ (%zu)", out_cz.size());
     }
     std::printf("facade_test: %s (%d failures)\n", failures ? "FAIL" : "ok", failures);
     return failures ? 1 : 0;

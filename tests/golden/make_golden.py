@@ -3,6 +3,7 @@
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py work     # work_totals.json only
     python tests/golden/make_golden.py frame    # smallpt9_1024x768x64.json only
+    python tests/golden/make_golden.py literals # literals.json only
 
 The reference holds no golden vectors for this path and cannot be built
 here, so the fixtures come from computations independent of the oracle's
@@ -19,9 +20,10 @@ block interpreter and of the GPU backend:
 * smallpt9_1024x768x64.json — the whole path-tracing workload frame
   (50,331,648 sample threads) from the direct C++ restatement: SHA-256 of
   each radiance channel's bits and a sample of (sample index, bits) rows.
-* literals.json — literal strings with the correctly rounded float32 and
-  float64 values (Python's parser) beside the values of the reference's
-  literal parser as restated by the oracle.
+* literals.json — literal strings with the value of the reference's literal
+  parser from a Python restatement in exact rational arithmetic
+  (tests/literal_ref.py, not the C++ restatement the oracle and the product
+  share) and the correctly rounded float32 and float64 values.
 * work_totals.json — the algorithmic work of whole configurations (thread
   block steps, ops = #Compute + 1 per block run, floating-point ops with
   fma = 2): fib and Collatz in closed form from the IR text and the
@@ -188,25 +190,31 @@ LITERALS = ["0", "1", "0.5", "0.1", "0.25", "2.5", "1e-12", "1e-6", "1e5", "1e30
 
 
 def literal_fixture():
-    from tests import oracle_lib as O
+    """Literal texts with two values each, neither from the product's or the
+    oracle's parser: s2f_* from the Python restatement of the reference's
+    parser (tests/literal_ref.py, exact rationals rounded at every step) and
+    rn_* the correctly rounded value."""
+    from tests import literal_ref as R
     out = []
     for t in LITERALS:
-        f32 = O.parse("f32", t)
-        f64 = O.parse("f64", t)
-        out.append({"text": t,
-                    "ref_f32_bits": struct.unpack("<I", struct.pack("<f", f32))[0],
-                    "ref_f64_bits": struct.unpack("<Q", struct.pack("<d", f64))[0],
-                    "rn_f32_bits": struct.unpack("<I", struct.pack("<f", float(np.float32(t))))[0],
-                    "rn_f64_bits": struct.unpack("<Q", struct.pack("<d", float(t)))[0]})
+        row = {"text": t}
+        for kind in ("f32", "f64"):
+            row[f"s2f_{kind}_bits"] = R.bits(R.s2f(t, kind), kind)
+            row[f"rn_{kind}_bits"] = R.bits(R.correctly_rounded(t, kind), kind)
+        out.append(row)
     return {"literals": out,
-            "how": "ref_*: the reference literal parser as restated by the oracle; "
-                   "rn_*: correctly rounded (Python/numpy parsers)"}
+            "how": "s2f_*: the reference parser's value (string.hpp:312-352) from the Python "
+                   "restatement tests/literal_ref.py; rn_*: correctly rounded"}
 
 
 def main():
     if sys.argv[1:] == ["frame"]:
         (HERE / "smallpt9_1024x768x64.json").write_text(json.dumps(frame_fixture(), indent=0) + "\n")
         print("frame", flush=True)
+        return
+    if sys.argv[1:] == ["literals"]:
+        (HERE / "literals.json").write_text(json.dumps(literal_fixture(), indent=0) + "\n")
+        print("literals", flush=True)
         return
     if sys.argv[1:] == ["work"]:
         (HERE / "work_totals.json").write_text(json.dumps(work_fixture(), indent=1) + "\n")

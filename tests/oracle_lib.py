@@ -38,6 +38,9 @@ def lib(path=PATH):
                 fn.restype = t
         L.oracle_smallpt9_direct.argtypes = [C.c_void_p] * 5 + [C.c_uint64, C.c_uint64, C.c_int]
         L.oracle_parse_u64.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
+        L.oracle_collatz_direct.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
+        L.oracle_kernel_literal.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t,
+                                            C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
         L.oracle_parse_i32.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
         L.oracle_trace.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint32,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
@@ -73,6 +76,35 @@ def run(name, inputs, threads=8, path=PATH):
     rc = lib(path).oracle_run(k, ip, op, 0, n, threads, C.byref(ex), C.byref(er))
     assert rc == 0
     return outs, er.value
+
+
+def literals(name, path=PATH):
+    """(text, dtype, bits) of every literal operand of kernel `name`, in
+    statement order, with the bits the kernel computes with."""
+    L = lib(path)
+    k = L.oracle_kernel_find(name.encode())
+    if k < 0:
+        raise KeyError(name)
+    out, buf, ty, bits = [], C.create_string_buffer(256), C.c_int(), C.c_uint64()
+    i = 0
+    while L.oracle_kernel_literal(k, i, buf, 256, C.byref(ty), C.byref(bits)) == 0:
+        out.append((buf.value.decode(), DTYPES[ty.value], bits.value))
+        i += 1
+    return out
+
+
+def collatz_direct(x, threads=8):
+    """Collatz step counts by direct iteration of the definition."""
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    out = np.zeros(x.size, np.uint32)
+    rc = lib().oracle_collatz_direct(x.ctypes.data, out.ctypes.data, x.size, threads)
+    assert rc == 0, "a trajectory left 64 bits"
+    return out
+
+
+def kernel_names(path=PATH):
+    L = lib(path)
+    return [L.oracle_kernel_name(k).decode() for k in range(L.oracle_kernel_count())]
 
 
 def parse(kind, text):

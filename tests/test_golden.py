@@ -59,23 +59,36 @@ def test_smallpt9_frame_oracle_interpreter():
 
 
 def test_literals_table():
+    """The oracle's parser (the C++ restatement the product shares) against
+    the fixture's values from the Python restatement of the reference
+    parser, bit for bit, in both precisions."""
     f = load("literals.json")
     for row in f["literals"]:
         t = row["text"]
         f32 = O.parse("f32", t)
         f64 = O.parse("f64", t)
-        assert struct.unpack("<I", struct.pack("<f", f32))[0] == row["ref_f32_bits"], t
-        assert struct.unpack("<Q", struct.pack("<d", f64))[0] == row["ref_f64_bits"], t
+        assert struct.unpack("<I", struct.pack("<f", f32))[0] == row["s2f_f32_bits"], t
+        assert struct.unpack("<Q", struct.pack("<d", f64))[0] == row["s2f_f64_bits"], t
         # the reference parser stays within a few ulp of correct rounding
-        assert abs(row["ref_f32_bits"] - row["rn_f32_bits"]) <= 4, t
-        assert abs(row["ref_f64_bits"] - row["rn_f64_bits"]) <= 8, t
+        assert abs(row["s2f_f32_bits"] - row["rn_f32_bits"]) <= 4, t
+        assert abs(row["s2f_f64_bits"] - row["rn_f64_bits"]) <= 8, t
+
+
+def test_literals_table_is_not_correct_rounding():
+    """The fixture separates the reference's parser from a correctly rounding
+    one: a parser that rounded correctly would fail the table above on
+    these texts."""
+    f = load("literals.json")
+    inexact = {r["text"] for r in f["literals"]
+               if r["s2f_f32_bits"] != r["rn_f32_bits"] or r["s2f_f64_bits"] != r["rn_f64_bits"]}
+    assert {"0.093073", "0.5135", "0.999"} <= inexact
 
 
 @pytest.mark.parametrize("text,exact", [("0.5", True), ("0.25", True), ("12", True),
                                         ("16777216", True), ("1e5", True)])
 def test_literals_exact_where_representable(text, exact):
     row = next(r for r in load("literals.json")["literals"] if r["text"] == text)
-    assert (row["ref_f32_bits"] == row["rn_f32_bits"]) == exact
+    assert (row["s2f_f32_bits"] == row["rn_f32_bits"]) == exact
 
 
 def test_ppm_of_fixture_frame(tmp_path):

@@ -221,6 +221,78 @@ def test_literal_inexact_within_ulps_of_strtof():
         assert ulps64 <= 8, (t, got64)
 
 
+# the reference's own checks of its float parser (src/tests/string.cpp:40-56):
+# the parsed value equals the compiler's (correctly rounded) literal
+REFERENCE_S2F_CHECKS = [("f32", t) for t in ("0", "1", "1234.567", "1e2", "0.5e2", "0.5e-2",
+                                             "1e-8")] + \
+                       [("f64", t) for t in ("0", "-1234.567", "-1e2", "-0.5e2", "-0.5e-2",
+                                             "-2430.5e4")]
+
+
+@pytest.mark.parametrize("kind,text", REFERENCE_S2F_CHECKS)
+def test_literal_reference_checks(kind, text):
+    from tests import literal_ref as R
+    got = O.parse(kind, text)
+    assert got is not None
+    assert float(got) == float(R.correctly_rounded(text, kind)), (kind, text, got)
+    # and the independent restatement agrees with the shared C++ one
+    assert float(got) == float(R.s2f(text, kind))
+
+
+@pytest.mark.parametrize("text", ["e334g", "-", "+"])
+def test_literal_reference_rejects(text):
+    from tests import literal_ref as R
+    assert O.parse("f32", text) is None
+    assert R.s2f(text, "f32") is None
+
+
+def _int_literal(text, dtype):
+    """Integer literal value as the reference's s2i_base_prefix_fn reads it
+    (string.hpp:286-306): optional sign for signed types, 0x / 0b prefixes."""
+    signed = np.issubdtype(dtype, np.signedinteger)
+    neg = False
+    if signed and text[:1] in "+-":
+        neg, text = text[0] == "-", text[1:]
+    base = 10
+    if text[:2] in ("0x", "0X"):
+        base, text = 16, text[2:]
+    elif text[:2] in ("0b", "0B"):
+        base, text = 2, text[2:]
+    v = int(text, base)
+    bits = 8 * np.dtype(dtype).itemsize
+    return (-v if neg else v) % (1 << bits)
+
+
+def _check_kernel_literals(path):
+    from tests import literal_ref as R
+    seen = 0
+    for name in O.kernel_names(path):
+        for text, dt, bits in O.literals(name, path):
+            if dt in (np.float32, np.float64):
+                kind = "f32" if dt == np.float32 else "f64"
+                want = R.bits(R.s2f(text, kind), kind)
+            else:
+                want = _int_literal(text, dt)
+            assert bits == want, (name, text, dt, hex(bits), hex(want))
+            seen += 1
+    return seen
+
+
+def test_every_kernel_literal_matches_restatement():
+    """Every literal operand of every kernel of the library (the constants
+    the GPU computes with come from the same lowering) against the Python
+    restatement of the reference's parser."""
+    assert _check_kernel_literals(O.PATH) > 300
+
+
+def test_reference_kernel_literals_match_restatement():
+    """The same over the reference's own kernels (path tracers, test
+    programs) where they were built from its checkout."""
+    if not O.REF_PATH.exists():
+        pytest.skip("reference kernels not built (no checkout)")
+    assert _check_kernel_literals(O.REF_PATH) > 500
+
+
 def test_integer_literals():
     assert O.parse("u64", "0x20") == 32
     assert O.parse("u64", "0b11") == 3
