@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define LOOM_MAX_IO 16
+#define LOOM_MAX_IO 32
 #define LOOM_MAX_BLOCKS 64
 #define LOOM_NUM_COUNTERS 16
 

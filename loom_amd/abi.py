@@ -11,7 +11,7 @@ from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_build" / "libloom_amd.so"
 
-MAX_IO = 16
+MAX_IO = 32
 MAX_BLOCKS = 64
 NUM_COUNTERS = 16
 

@@ -34,7 +34,7 @@
 
 namespace loom::hipvm
 {
-inline constexpr int         kMaxIO    = 16;
+inline constexpr int         kMaxIO    = 32;
 inline constexpr std::size_t kNumStats = 13;
 /// Counter copies: a wave adds into copy (workgroup index mod kStatSlots),
 /// the host sums the copies, so hundreds of thousands of waves do not

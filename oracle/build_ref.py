@@ -3,8 +3,9 @@ kernels and test programs (oracle/ref_kernels.hpp) against this repository's
 front-end, from the reference checkout where it lies, into oracle/_ref/:
 
 * liboracle_ref.so — the oracle interpreter (oracle.cpp) over those kernels,
-  and the reference's own validators of the test programs
-  (ref_validators.cpp);
+  the reference's own validators of the test programs (ref_validators.cpp),
+  and the C++ side of its path-tracing and vector macro checks
+  (ref_macro_checks.cpp);
 * libloom_refkernels.so — the same kernels lowered for gfx950 by the
   product's backend; loading it (after the product library) registers them
   with the product library, where the C ABI finds them by name;
@@ -31,8 +32,9 @@ OUT = HERE / "_ref"
 ORACLE_LIB = OUT / "liboracle_ref.so"
 KERNEL_LIB = OUT / "libloom_refkernels.so"
 FACADE_BIN = OUT / "ref_facade_test"
+CLANG = os.environ.get("LOOM_HOST_CLANG", "/opt/rocm/llvm/bin/clang++")
 SOURCES = ["ref_pt_const.hip", "ref_pt2_const.hip", "ref_pt2_const_1024.hip",
-           "ref_programs_a.hip", "ref_programs_b.hip", "ref_programs_c.hip"]
+           "ref_programs_a.hip", "ref_programs_b.hip", "ref_programs_c.hip", "ref_macros.hip"]
 
 
 def _run(cmd):
@@ -57,12 +59,22 @@ def build(ref: Path, before_link=None, oracle_only: bool = False) -> list[Path]:
     OUT.mkdir(parents=True, exist_ok=True)
 
     def oracle():
+        # the C++ side of the macro checks is compiled as the reference
+        # compiles its own (clang, contraction of a*b+c within an expression,
+        # a CPU with fused multiply-add: CMakeLists.txt:64-77), the
+        # interpreter and the validators with the op contract's flags
+        checks = OUT / "ref_macro_checks.o"
+        _run([CLANG, "-O2", "-std=c++20", "-fPIC", "-march=x86-64-v3", "-ffp-contract=on",
+              f"-I{HERE}", f"-I{src}", f"-I{ext}", "-c", str(HERE / "ref_macro_checks.cpp"),
+              "-o", str(checks)])
         tmp = ORACLE_LIB.with_suffix(".tmp")
         _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",
               "-fno-fast-math", "-DLOOM_ORACLE_REF", f"-I{ROOT / 'include' / 'loom_vm'}",
               f"-I{ROOT / 'include'}", f"-I{HERE}", f"-I{src}", f"-I{ext}", "-shared", "-o",
-              str(tmp), str(HERE / "oracle.cpp"), str(HERE / "ref_validators.cpp"), "-lpthread"])
+              str(tmp), str(HERE / "oracle.cpp"), str(HERE / "ref_validators.cpp"), str(checks),
+              "-lpthread"])
         os.replace(tmp, ORACLE_LIB)
+        checks.unlink()
         return ORACLE_LIB
 
     def obj(name):

@@ -26,6 +26,8 @@
 #include <smallpt/vm/pt.hpp>
 #include <tests/programs.hpp>
 
+#include "ref_macros.hpp"
+
 namespace loom_ref
 {
 using c_b1_t0001_d2  = loom::vm::constants<CS(40), loom::sl2cs<"0.0001">, CS(64),
@@ -122,4 +124,4 @@ KERNEL_END(IVAR(w0), IVAR(w1), IVAR(c0), IVAR(c1), TVAR(u0), TVAR(u1));
 
 #define LOOM_REF_CATALOG(X)                                                                       \
     LOOM_REF_CATALOG_PT_CONST(X) LOOM_REF_CATALOG_PT2_CONST(X) LOOM_REF_CATALOG_PT2_CONST_1024(X) \
-    LOOM_REF_CATALOG_PROGRAMS(X)
+    LOOM_REF_CATALOG_PROGRAMS(X) LOOM_REF_CATALOG_MACROS(X)

@@ -59,7 +59,7 @@ def io(name, path=PATH):
     if k < 0:
         raise KeyError(name)
     ni, no, nr = C.c_int(), C.c_int(), C.c_int()
-    it, ot = (C.c_int * 16)(), (C.c_int * 16)()
+    it, ot = (C.c_int * 32)(), (C.c_int * 32)()
     L.oracle_kernel_io(k, C.byref(ni), C.byref(no), it, ot, C.byref(nr))
     return k, [DTYPES[it[i]] for i in range(ni.value)], [DTYPES[ot[i]] for i in range(no.value)]
 
@@ -70,8 +70,8 @@ def run(name, inputs, threads=8, path=PATH):
     ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
     n = len(ins[0])
     outs = [np.zeros(n, t) for t in otys]
-    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
-    op = (C.c_void_p * 16)(*[a.ctypes.data for a in outs])
+    ip = (C.c_void_p * 32)(*[a.ctypes.data for a in ins])
+    op = (C.c_void_p * 32)(*[a.ctypes.data for a in outs])
     ex, er = C.c_uint64(), C.c_uint32()
     rc = lib(path).oracle_run(k, ip, op, 0, n, threads, C.byref(ex), C.byref(er))
     assert rc == 0
@@ -131,7 +131,7 @@ def trace(name, inputs, thread, max_steps):
     L = lib()
     ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
     ni, no, nr = C.c_int(), C.c_int(), C.c_int()
-    it, ot = (C.c_int * 16)(), (C.c_int * 16)()
+    it, ot = (C.c_int * 32)(), (C.c_int * 32)()
     L.oracle_kernel_io(k, C.byref(ni), C.byref(no), it, ot, C.byref(nr))
     names = []
     buf = C.create_string_buffer(256)
@@ -142,7 +142,7 @@ def trace(name, inputs, thread, max_steps):
     regs = np.zeros((max_steps, nr.value), np.uint64)
     errs = np.zeros(max_steps, np.uint32)
     steps = C.c_uint32()
-    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
+    ip = (C.c_void_p * 32)(*[a.ctypes.data for a in ins])
     assert L.oracle_trace(k, ip, thread, max_steps, first.ctypes.data, regs.ctypes.data,
                           errs.ctypes.data, C.byref(steps)) == 0
     out = []
@@ -157,7 +157,7 @@ def work(name, inputs, threads=8):
     """(thread steps, ops, flops) of every thread of `inputs` (oracle)."""
     k, itys, _ = io(name)
     ins = [np.ascontiguousarray(a, dtype=t) for a, t in zip(inputs, itys)]
-    ip = (C.c_void_p * 16)(*[a.ctypes.data for a in ins])
+    ip = (C.c_void_p * 32)(*[a.ctypes.data for a in ins])
     st, op, fl = C.c_uint64(), C.c_uint64(), C.c_uint64()
     assert lib().oracle_work(k, ip, 0, len(ins[0]), threads, C.byref(st), C.byref(op),
                              C.byref(fl)) == 0
