@@ -228,6 +228,19 @@ int loom_host_free(void* ptr);
 int loom_run(int kernel, int mode, const void* const* inputs,
              void* const* outputs, uint64_t n, const loom_params* params,
              loom_run_stats* stats);
+/* loom_run without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The hybrid form's
+ * control block and regroup queues live in the library's static device
+ * scratch array of 16 MiB (one per GPU, held until the process ends); a
+ * regrouping run whose queues (2 x queue capacity x queue_row_bytes) do not
+ * fit is rejected with LOOM_E_CAPACITY.  Passes of a regrouping run are timed
+ * only as a whole (ms_main = ms_total).  Every call on one GPU shares the
+ * scratch array: calls that overlap on one GPU (from several threads) race
+ * and must be serialised by the caller.  Inputs, outputs and other
+ * arguments as loom_run, on the default context. */
+int loom_run_noalloc(int kernel, int mode, const void* const* inputs, void* const* outputs,
+                     uint64_t n, const loom_params* params, loom_run_stats* stats);
 /* The same with host columns (copies in and out; device buffers cached). */
 int loom_run_host(int kernel, int mode, const void* const* inputs,
                   void* const* outputs, uint64_t n, const loom_params* params,

@@ -188,6 +188,9 @@ SIGNATURES = {
     "loom_host_free": (C.c_int, [C.c_void_p]),
     "loom_run": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                            C.c_uint64, C.POINTER(Params), C.POINTER(RunStats)]),
+    "loom_run_noalloc": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
+                                   C.POINTER(RunStats)]),
     "loom_run_host": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p),
                                 C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(Params),
                                 C.POINTER(RunStats)]),

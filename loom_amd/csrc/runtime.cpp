@@ -41,6 +41,15 @@ batch_registry()
 
 thread_local std::string g_error;
 
+/// Device memory of loom_run_noalloc: the hybrid control block at the start,
+/// the two regroup queue sides after it.  One copy per GPU, held until the
+/// process ends; host code uses g_scratch (its device address, looked up
+/// at loom_init), never the array's name.
+constexpr std::size_t kScratchBytes = std::size_t(16) << 20;
+constexpr std::size_t kScratchCtl   = (sizeof(HybridCtl) + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void*                                           g_scratch = nullptr;
+
 Device        g_dev;
 std::mutex    g_init_mutex;
 loom_context* g_default = nullptr;
@@ -191,7 +200,7 @@ hybrid_qcap(const loom_params* p, unsigned grid, std::uint64_t n)
 /// block and queue sides) on @p x's stream.
 int
 run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy,
-              unsigned grid, const HybridArgs& h, loom_run_stats* st)
+              unsigned grid, const HybridArgs& h, loom_run_stats* st, bool fixed_events = false)
 {
     const std::uint32_t nb = static_cast<std::uint32_t>(e.info.n_blocks);
     LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
@@ -205,6 +214,8 @@ run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c
     std::size_t ev_used = 0;
     auto next_ev = [&]() -> hipEvent_t
     {
+        // without event creation: one timed pass on the context's own events
+        if(fixed_events) return ev_used < 2 ? x.ev[2 + ev_used++] : nullptr;
         if(ev_used == x.pass_ev.size())
         {
             hipEvent_t ev{};
@@ -233,12 +244,17 @@ run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c
         {
             for(std::uint32_t k = 0; k < check_every; ++k)
             {
-                hipEvent_t a = next_ev(), b = next_ev();
-                if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
-                LOOM_HIP(hipEventRecord(a, x.stream));
-                LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, x.stream));
-                LOOM_HIP(hipEventRecord(b, x.stream));
-                evs.emplace_back(a, b);
+                if(fixed_events)
+                    LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, x.stream));
+                else
+                {
+                    hipEvent_t a = next_ev(), b = next_ev();
+                    if(!a || !b) return fail(LOOM_E_RUNTIME, "event creation failed");
+                    LOOM_HIP(hipEventRecord(a, x.stream));
+                    LOOM_HIP(e.hybrid_pass[policy](io, c, h, grid, x.stream));
+                    LOOM_HIP(hipEventRecord(b, x.stream));
+                    evs.emplace_back(a, b);
+                }
                 hipLaunchKernelGGL(hybrid_setup_kernel, dim3(1), dim3(64), 0, x.stream, h.ctl, nb,
                                    h.qcap, c.n, 0u);
                 LOOM_HIP(hipGetLastError());
@@ -265,11 +281,13 @@ run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c
         LOOM_HIP(hipEventElapsedTime(&ms, a, b));
         main_ms += ms;
     }
+    // passes timed only as a whole (fixed events, regrouping run)
+    if(evs.empty()) main_ms = total;
     if(st)
     {
         st->ms_total      = total;
         st->ms_main       = main_ms;
-        st->main_launches = static_cast<std::uint32_t>(evs.size());
+        st->main_launches = evs.empty() ? passes : static_cast<std::uint32_t>(evs.size());
         st->passes        = passes;
         st->grid_waves    = grid;
     }
@@ -310,9 +328,37 @@ run_hybrid(loom_context& x, int kidx, const KernelEntry& e, const IoArgs& io, Ct
     return run_hybrid_on(x, e, io, c, policy, grid, h, st);
 }
 
+/// @brief A hybrid run whose control block and queues are the scratch
+/// array (no allocation): fails with LOOM_E_CAPACITY when the queues of a
+/// regrouping run do not fit.
+int
+run_hybrid_scratch(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c,
+                   const loom_params* p, loom_run_stats* st)
+{
+    if(!g_scratch) return fail(LOOM_E_NODEVICE, "scratch memory not set up");
+    const int      policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
+    const unsigned grid   = hybrid_grid(e, policy, c.regroup != 0, p, c.n);
+    std::uint64_t  qcap   = hybrid_qcap(p, grid, c.n);
+    char*          base   = static_cast<char*>(g_scratch);
+    HybridArgs     h;
+    h.ctl  = reinterpret_cast<HybridCtl*>(base);
+    h.qcap = qcap;
+    if(c.regroup)
+    {
+        const std::uint64_t side = qcap * e.info.queue_row_bytes;
+        if(2 * side > kScratchBytes - kScratchCtl)
+            return fail(LOOM_E_CAPACITY,
+                        "regroup queues need %llu bytes, the scratch array holds %zu",
+                        static_cast<unsigned long long>(2 * side), kScratchBytes - kScratchCtl);
+        h.q[0] = base + kScratchCtl;
+        h.q[1] = base + kScratchCtl + side;
+    }
+    return run_hybrid_on(x, e, io, c, policy, grid, h, st, true);
+}
+
 int
 run_on(loom_context& x, int kernel, int mode, const void* const* inputs, void* const* outputs,
-       std::uint64_t n, const loom_params* params, loom_run_stats* stats)
+       std::uint64_t n, const loom_params* params, loom_run_stats* stats, bool scratch = false)
 {
     if(int r = check_kernel(kernel)) return r;
     const KernelEntry& e = registry()[kernel];
@@ -325,6 +371,8 @@ run_on(loom_context& x, int kernel, int mode, const void* const* inputs, void* c
     int r = 0;
     if(mode == LOOM_MODE_COMPACT)
         r = run_compact(x, e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
+    else if(scratch)
+        r = run_hybrid_scratch(x, e, io, c, params, stats);
     else
         r = run_hybrid(x, kernel, e, io, c, params, stats);
     if(r) return r;
@@ -704,6 +752,9 @@ loom_init(int device)
     LOOM_HIP(hipGetDeviceProperties(&prop, device));
     g_dev.id  = device;
     g_dev.cus = prop.multiProcessorCount;
+    // the device address of this GPU's scratch array (host code must not
+    // pass the array's own name to a kernel)
+    LOOM_HIP(hipGetSymbolAddress(&g_scratch, HIP_SYMBOL(scratch)));
     auto x    = std::make_unique<loom_context>();
     if(int r = context_init(*x))
     {
@@ -1051,6 +1102,15 @@ loom_run(int kernel, int mode, const void* const* inputs, void* const* outputs, 
          const loom_params* params, loom_run_stats* stats)
 {
     return loom_ctx_run(nullptr, kernel, mode, inputs, outputs, n, params, stats);
+}
+
+int
+loom_run_noalloc(int kernel, int mode, const void* const* inputs, void* const* outputs,
+                 uint64_t n, const loom_params* params, loom_run_stats* stats)
+{
+    Locked l;
+    if(int r = lock_context(nullptr, l)) return r;
+    return run_on(*l.x, kernel, mode, inputs, outputs, n, params, stats, true);
 }
 
 int
