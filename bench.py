@@ -7,8 +7,9 @@ A step renders one whole frame (50,331,648 sample threads) with inputs
 resident in HBM.  The JSON line reports the hybrid throughput as `value`,
 the compact throughput and active-lane fractions beside it, the roofline of
 the dominant kernel (hybrid pass kernel, HIP events around each launch) and
-the CPU baseline (the oracle's direct C++ path tracer on the host cores, a
-bounded sample of the same frame).
+the CPU baseline (the oracle's block interpreter of the same IR kernel, the
+reference's scalar VM backend restated, on the host threads of one GPU, a
+bounded sample of the same frame; the direct C++ path tracer beside it).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -26,10 +27,24 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# Every execution context of the library has a kernel stream and a copy
+# stream, and HIP spreads streams over GPU_MAX_HW_QUEUES hardware queues (4
+# by default), sharing queues once there are more streams.  With 4, the
+# serving leg run after the batch leg served 2,420 of 3,000 requests/s at a
+# p50 of 50 ms (GPU busy 41%): its copies no longer overlapped its kernels.
+# With 16 it serves 3,078 requests/s at a p50 of 2.1 ms.  Set before HIP
+# initialises.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 METRIC = ("path-trace samples/sec + active-lane % (hybrid vs compact GPU); "
           "p99 req latency")
-FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, published
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 vector, published (counts packed FP32)
+# one non-packed FP32 fma per lane and cycle: 256 CUs x 4 SIMDs x 16 lanes x
+# 2 flops x 2.4 GHz (half the published figure; packed FP32 is not used)
+FP32_NONPACKED_TFLOPS = 256 * 4 * 16 * 2 * 2.4e9 / 1e12
 HBM_PEAK_GBPS = 8000.0    # MI355X HBM3E, published
+CPU_THREADS = 16          # host threads of one GPU's share of the test box
 POLICY_NAMES = {0: "successor-major", 1: "topo", 2: "sweep"}
 W, H, SPP = 1024, 768, 64
 SERVE_QPS = 1000.0
@@ -90,24 +105,44 @@ def mixed_batch(steps):
             "outputs_identical_to_separate": same, "params": bp}
 
 
+SERVE_CHECKED = 24  # requests per serving run whose outputs are compared
+
+
 def serving(qps, count):
-    """A Poisson stream of mixed requests served with continuous batching."""
-    from loom_amd import serve
+    """A Poisson stream of mixed requests served with continuous batching.
+    Most requests write into shared pinned sinks; a seeded sample of
+    SERVE_CHECKED requests gets output columns of its own, compared after
+    the run with the same launch run alone (every request of a smaller trace
+    is compared in the GPU tests)."""
+    import loom_amd
+    from loom_amd import abi, serve
     pools = serve.Pools.make()
     specs = serve.poisson_trace(qps=qps, count=count)
-    # outputs land in shared pinned sinks (this run does not read them;
-    # the serving tests check every request's bits)
     sinks = {"smallpt9_f32": [serve.Pinned(serve.PT_MAX, np.float32) for _ in range(3)],
              "fib_u64": [serve.Pinned(1 << serve.INT_MAX_LOG2, np.uint64)],
              "collatz_u64": [serve.Pinned(1 << serve.INT_MAX_LOG2, np.uint32)]}
     outs = [[p.array[:s.n] for p in sinks[s.kernel]] for s in specs]
+    checked = sorted(np.random.default_rng(int(qps)).choice(
+        np.arange(8, len(specs)), min(SERVE_CHECKED, len(specs) - 8), replace=False).tolist())
+    own = {}
+    for i in checked:
+        k = loom_amd.kernel(specs[i].kernel)
+        own[i] = [serve.Pinned(specs[i].n, dt) for dt in k.outputs]
+        outs[i] = [p.array for p in own[i]]
     srv = serve.Server(hbm_budget=SERVE_BUDGET, max_resident=32, step_threads=SERVE_STEP)
     # warm the path (first launches, page-in of the pools)
     srv.run(specs[:8], pools, realtime=False, outputs=outs[:8])
     reqs, st, _ = srv.run(specs, pools, realtime=True, outputs=outs)
     srv.close()
+    same = all(q.status == 0 for q in reqs)
+    for i in checked:
+        alone = loom_amd.run(specs[i].kernel, abi.MODE_HYBRID,
+                             pools.inputs_of(specs[i], full=True)).outputs
+        same = same and all(np.array_equal(g.view(np.uint8), a.view(np.uint8))
+                            for g, a in zip(outs[i], alone))
     summ = serve.latency_summary(reqs, st)
     summ.update({"qps": qps, "hbm_budget": SERVE_BUDGET, "step_threads": SERVE_STEP,
+                 "outputs_checked": len(checked), "outputs_identical_to_alone": same,
                  "workload": "Poisson arrivals; half frames 128^2-512^2 at 4-16 spp, half "
                              "fib/Collatz of 2^14-2^20 threads"})
     return summ
@@ -154,44 +189,51 @@ def traffic_from_profile():
     return None
 
 
-def valu_ceiling_from_profile():
-    """Measured non-packed FP32 fma rate (TFLOP/s) of the committed VALU probe
-    summary; None if it is absent."""
-    p = ROOT / "profiles" / "valu_rate_probe.txt"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        f = line.split()
-        if line.strip().startswith("fp32 fma") and "TFLOP/s" in f:
-            return float(f[f.index("TFLOP/s") - 1])
-    return None
-
-
 def cpu_baseline():
-    """The oracle's direct C++ path tracer (compiled scalar restatement),
-    one thread per host core (at most 16), on the first half of the frame
-    (about 30 CPU-seconds)."""
+    """The reference's CPU VM path as restated by the oracle: its block
+    interpreter (the scalar backend's semantics, src/vm/bse.hpp:139-319)
+    running the same IR kernel, on the GPU's share of host threads, over
+    the first 32 rows of the frame (2,097,152 samples, about 10 s of CPU
+    work); beside it the oracle's direct C++ path tracer (the same
+    per-sample algorithm written directly in C++, no VM), and one-thread
+    rates of both."""
     from tests import oracle_lib
     from loom_amd import workloads
-    n = W * H * SPP // 2
-    threads = min(os.cpu_count() or 1, 16)
+    job_word = workloads.pt_job(W, H, SPP)
+    n = 32 * W * SPP
     sid = np.arange(n, dtype=np.uint32)
-    job = np.full(n, workloads.pt_job(W, H, SPP), np.uint32)
+    job = np.full(n, job_word, np.uint32)
+    threads = CPU_THREADS
     t = time.perf_counter()
-    oracle_lib.smallpt9_direct(sid, job, threads=threads)
+    oracle_lib.run("smallpt9_f32", [sid, job], threads=threads)
     dt = time.perf_counter() - t
-    # one core on the first 1/16 of that sample, comparable to single-core figures
-    n1 = n // 16
+    n1 = n // 64
     t = time.perf_counter()
-    oracle_lib.smallpt9_direct(sid[:n1], job[:n1], threads=1)
+    oracle_lib.run("smallpt9_f32", [sid[:n1], job[:n1]], threads=1)
     dt1 = time.perf_counter() - t
+    # the direct tracer: the first half of the frame
+    nd = W * H * SPP // 2
+    sidd = np.arange(nd, dtype=np.uint32)
+    jobd = np.full(nd, job_word, np.uint32)
+    t = time.perf_counter()
+    oracle_lib.smallpt9_direct(sidd, jobd, threads=threads)
+    ddt = time.perf_counter() - t
+    nd1 = nd // 16
+    t = time.perf_counter()
+    oracle_lib.smallpt9_direct(sidd[:nd1], jobd[:nd1], threads=1)
+    ddt1 = time.perf_counter() - t
     return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} samples (rows 0-{H // 2 - 1}) of the {W}x{H}x{SPP} frame, "
-                      f"direct C++ restatement of the kernel, {threads} threads, "
-                      f"{dt:.2f} s wall",
+            "sample": f"first {n} samples (rows 0-31) of the {W}x{H}x{SPP} frame, the "
+                      f"oracle's block interpreter of the same IR kernel (the reference's "
+                      f"scalar VM backend restated), {threads} threads, {dt:.2f} s wall",
             "single_core": {"value": n1 / dt1, "unit": "samples/s",
                             "sample": f"first {n1} samples, 1 thread, {dt1:.2f} s wall"},
-            "cpu_model": cpu_model()}
+            "algorithm_only": {
+                "value": nd / ddt, "unit": "samples/s", "cores": threads,
+                "sample": f"first {nd} samples (rows 0-{H // 2 - 1}), the path tracer written "
+                          f"directly in C++ (no VM), {threads} threads, {ddt:.2f} s wall",
+                "single_core": nd1 / ddt1},
+            "cpu_model": cpu_model(), "host_threads_seen": os.cpu_count()}
 
 
 def cpu_model() -> str:
@@ -206,9 +248,9 @@ def cpu_model() -> str:
 
 def integer_configs(steps: int):
     """fib u64 over 2^20 threads and Collatz u64 over 2^24 threads, inputs in
-    HBM: the tuned hybrid form and the compact form under the reference
-    policy and the sweep policy, threads/s and active lanes, outputs
-    compared across forms."""
+    HBM: the form the autotuner chose, the tuned hybrid form and the compact
+    form under the reference policy and the sweep policy, threads/s and
+    active lanes, outputs compared across forms."""
     import loom_amd
     from loom_amd import abi, autotune, workloads
     out = {}
@@ -220,13 +262,15 @@ def integer_configs(steps: int):
         k = loom_amd.kernel(name)
         n = len(cols[0])
         din = [loom_amd.DeviceArray.from_numpy(c) for c in cols]
-        forms = {"hybrid": (abi.MODE_HYBRID, loom_amd.make_params(
+        tmode, tparams = autotune.best_form(name)
+        forms = {"tuned": autotune.params_of_form(tmode, tparams),
+                 "hybrid": (abi.MODE_HYBRID, loom_amd.make_params(
                      mode=abi.MODE_HYBRID, **autotune.best_params(name))),
                  "compact_successor_major": (abi.MODE_COMPACT, loom_amd.make_params(
                      mode=abi.MODE_COMPACT)),
                  "compact_sweep": (abi.MODE_COMPACT, loom_amd.make_params(
                      mode=abi.MODE_COMPACT, reconv=abi.RECONV_SWEEP))}
-        entry, first = {"threads": n}, None
+        entry, first = {"threads": n, "tuned_form": {"mode": tmode, "params": tparams}}, None
         for form, (mode, params) in forms.items():
             dout = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
             loom_amd.run(k, mode, din, dout, params=params)
@@ -316,15 +360,21 @@ def main():
                 for x, d in zip(h_img, f["outputs_sha256"]))
 
     if not a.no_compact:
-        steps(abi.MODE_COMPACT, cp, 1)
-        c_ms, c_last = steps(abi.MODE_COMPACT, cp, a.steps)
-        c_img = [o.numpy() for o in outs]
-        same = all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
-                   for x, y in zip(h_img, c_img))
-        result["compact"] = {"value": n / (c_ms / 1e3), "ms_per_step": c_ms,
-                             "active_lane_fraction": c_last.active_lane_fraction,
-                             "reconvergence": "successor-major (reference policy)"}
-        result["hybrid_over_compact"] = c_ms / h_ms
+        same = True
+        for key, params, label in (
+                ("compact", cp, "successor-major (reference policy)"),
+                ("compact_sweep", loom_amd.make_params(mode=abi.MODE_COMPACT,
+                                                       reconv=abi.RECONV_SWEEP),
+                 "sweep (every block in topological order)")):
+            steps(abi.MODE_COMPACT, params, 1)
+            c_ms, c_last = steps(abi.MODE_COMPACT, params, a.steps)
+            c_img = [o.numpy() for o in outs]
+            same = same and all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                                for x, y in zip(h_img, c_img))
+            result[key] = {"value": n / (c_ms / 1e3), "ms_per_step": c_ms,
+                           "active_lane_fraction": c_last.active_lane_fraction,
+                           "reconvergence": label}
+            result["hybrid_over_" + key] = c_ms / h_ms
         result["outputs_identical"] = same
 
     # roofline of the dominant kernel: the hybrid pass kernel
@@ -333,17 +383,18 @@ def main():
     flops = h_last.counters[abi.C_ACTIVE_FLOPS] / launches
     achieved = flops / (launch_ms / 1e3) / 1e12
     result["roofline"] = {
-        "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profile(),
         "kernel": f"hybrid_pass_kernel<smallpt9_f32, {POLICY_NAMES[hyb['reconv']]}>",
         "launch_ms": launch_ms, "launches_per_step": launches,
-        "measured_vector_fp32_ceiling": valu_ceiling_from_profile(),
-        "note": "VALU-bound (no MFMA in this path): useful FP32 flops of active lanes "
-                "(fma = 2) per launch over the launch time, against the FP32 vector peak; "
-                "measured_vector_fp32_ceiling is the non-packed fma rate of the VALU probe "
-                "(TFLOP/s)"}
-    ceil = result["roofline"]["measured_vector_fp32_ceiling"]
-    result["roofline"]["frac_of_measured_ceiling"] = achieved / ceil if ceil else None
+        "algorithmic_flops_per_launch": flops,
+        "peak_nonpacked": FP32_NONPACKED_TFLOPS,
+        "frac_of_nonpacked": achieved / FP32_NONPACKED_TFLOPS,
+        "note": "vector-ALU bound, no MFMA in this path: useful FP32 flops of active lanes "
+                "(fma = 2, device counter) per launch over the launch's HIP-event time; peak "
+                "is the published FP32 vector figure, which counts packed FP32; packed FP32 "
+                "is not used, so peak_nonpacked (one fma per lane and cycle) is the "
+                "attainable ceiling"}
 
     if not a.no_regroup:
         result["regroup"] = regroup_leg(k, ins, outs, h_img)
@@ -361,7 +412,8 @@ def main():
             hi = serving(SERVE_QPS_HIGH, a.requests)
             result["serving_high_load"] = {k: hi[k] for k in (
                 "qps", "requests_per_s", "p50_ms", "p99_ms", "max_ms", "gpu_busy",
-                "threads_per_s", "peak_resident")}
+                "threads_per_s", "peak_resident", "outputs_checked",
+                "outputs_identical_to_alone")}
 
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline()

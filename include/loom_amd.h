@@ -110,7 +110,17 @@ typedef struct loom_params
     uint64_t max_steps;      /* per-wave block dispatch budget; 0 = 2^40.
                                 Checked after every dispatch (reconv 0, 1)
                                 or after every sweep (reconv 2) */
+    uint32_t register_file;  /* compact: LOOM_REGFILE_VGPR (registers in
+                                VGPRs for the thread's life) or
+                                LOOM_REGFILE_MEMORY (one column per register
+                                in device memory, the reference's SoA
+                                register file, bve.hpp:494-693; each block
+                                loads what it reads and stores what stays
+                                live, so VGPRs hold one block's working set;
+                                sweep policy only) */
 } loom_params;
+#define LOOM_REGFILE_VGPR 0
+#define LOOM_REGFILE_MEMORY 1
 /* A NULL loom_params* means: threshold 0, refill on, regroup off, no stats,
  * successor-major for compact, sweep with live_threshold 48 for hybrid and
  * batched launches. */
@@ -147,8 +157,10 @@ int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
  * hybrid pass kernel; policy: loom_params.reconv, or'ed with
  * LOOM_RESOURCES_REGROUP for the hybrid pass kernel that regroups, which a
  * run with loom_params.regroup set launches; runs without it launch a pass
- * kernel compiled without the queue paths). */
+ * kernel compiled without the queue paths; or'ed with LOOM_RESOURCES_MEMORY
+ * for the compact kernel whose register file is in device memory). */
 #define LOOM_RESOURCES_REGROUP 0x100
+#define LOOM_RESOURCES_MEMORY 0x200
 int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
 /* Register r (0 <= r < n_registers) of a kernel: its name (NUL-terminated,
  * at most len bytes), first type (loom_type), width in bytes, number of

@@ -48,6 +48,9 @@ struct params
     std::uint32_t waves_per_cu   = 0;
     std::uint64_t queue_capacity = 0;
     std::uint64_t max_steps      = 0;
+    /// compact_function: keep the registers in device memory instead of
+    /// VGPRs (LOOM_REGFILE_MEMORY; runs under the sweep policy)
+    std::uint32_t register_file  = LOOM_REGFILE_VGPR;
 };
 
 namespace detail
@@ -85,8 +88,9 @@ to_c(const params& p, int mode)
     c.live_threshold = mode == LOOM_MODE_HYBRID ? p.live_threshold : 0;
     c.refill         = p.refill;
     c.regroup        = p.regroup;
+    c.register_file  = mode == LOOM_MODE_COMPACT ? p.register_file : LOOM_REGFILE_VGPR;
     c.reconv         = p.reconv >= 0 ? static_cast<std::uint32_t>(p.reconv) :
-                                       (mode == LOOM_MODE_HYBRID ? 2u : 0u);
+                       (mode == LOOM_MODE_HYBRID || c.register_file == LOOM_REGFILE_MEMORY ? 2u : 0u);
     c.waves_per_cu   = p.waves_per_cu;
     c.queue_capacity = p.queue_capacity;
     c.max_steps      = p.max_steps;

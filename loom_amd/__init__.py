@@ -167,11 +167,12 @@ class RunResult:
 
 def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, waves_per_cu=0,
                 queue_capacity=0, max_steps=0, stats=True, reconv=None,
-                mode=abi.MODE_COMPACT) -> abi.Params:
+                mode=abi.MODE_COMPACT, register_file=abi.REGFILE_VGPR) -> abi.Params:
     """Run parameters; `reconv` defaults to successor-major for the compact
     form (the reference policy) and to the sweep policy for hybrid,
     `live_threshold` to 48 for hybrid (0 for compact, which does not
-    refill)."""
+    refill); `register_file` = REGFILE_MEMORY keeps a compact run's
+    registers in device memory (sweep policy)."""
     p = abi.Params()
     if reconv is None:
         reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_SWEEP
@@ -186,6 +187,7 @@ def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, wa
     p.queue_capacity = int(queue_capacity)
     p.max_steps = int(max_steps)
     p.collect_stats = 1 if stats else 0
+    p.register_file = int(register_file)
     return p
 
 

@@ -19,6 +19,9 @@ MODE_COMPACT = 0
 MODE_HYBRID = 1
 # or'ed into loom_kernel_resources' policy: the pass kernel that regroups
 RESOURCES_REGROUP = 0x100
+RESOURCES_MEMORY = 0x200
+REGFILE_VGPR = 0
+REGFILE_MEMORY = 1
 
 RECONV_SUCCESSOR_MAJOR = 0
 RECONV_TOPO = 1
@@ -75,6 +78,7 @@ class Params(C.Structure):
         ("collect_stats", C.c_uint32),
         ("queue_capacity", C.c_uint64),
         ("max_steps", C.c_uint64),
+        ("register_file", C.c_uint32),
     ]
 
 

@@ -1,13 +1,16 @@
-"""Autotuner for the hybrid backend: picks the live-lane refill threshold,
-the regroup switch, the reconvergence policy and the persistent grid
-(waves per CU) by timing candidates on a sample of the real workload.
+"""Autotuner: picks the execution form of a kernel (the hybrid form or the
+predicated compact form) and its settings by timing candidates on a sample
+of the real workload.  Hybrid candidates vary the live-lane refill
+threshold, the regroup switch, the reconvergence policy and the persistent
+grid (waves per CU); compact candidates vary the reconvergence policy.
 
     python -m loom_amd.autotune smallpt9_f32      # tunes, writes tuned.json
     python -m loom_amd.autotune batch:pt_int_mix  # the 64-job mixed batch
 
-Results are stored per kernel in loom_amd/tuned.json; `best_params` reads
-them (or returns the defaults) so benchmarks and batched launches start from
-measured settings.
+Results are stored per kernel in loom_amd/tuned.json: `best_form` gives the
+fastest form that keeps the required share of lanes active, `best_params`
+the best hybrid settings (batched launches and the serving loop run the
+hybrid form), or the defaults for a kernel never tuned.
 """
 from __future__ import annotations
 
@@ -36,6 +39,9 @@ SPACE = {
     "waves_per_cu": [0, 8, 16, 20],
 }
 
+# the compact form: one wave per 64 threads, reconvergence policy only
+COMPACT_SPACE = {"reconv": [0, 1, 2]}
+
 
 def best_params(kernel_name: str) -> dict:
     """Tuned hybrid parameters of `kernel_name` (defaults if never tuned)."""
@@ -46,11 +52,34 @@ def best_params(kernel_name: str) -> dict:
     return dict(DEFAULT)
 
 
+def best_form(kernel_name: str) -> tuple[str, dict]:
+    """The tuned form of `kernel_name`, ("hybrid" | "compact", keyword
+    arguments of make_params); the hybrid form with its best settings when
+    the kernel was tuned without compact candidates or never tuned."""
+    if TUNED.exists():
+        data = json.loads(TUNED.read_text())
+        form = data.get(kernel_name, {}).get("form")
+        if form:
+            return form["mode"], dict(form["params"])
+    return "hybrid", best_params(kernel_name)
+
+
+def params_of_form(mode: str, params: dict):
+    """(abi mode, loom_params) of a form returned by best_form."""
+    import loom_amd
+    from loom_amd import abi
+    m = abi.MODE_HYBRID if mode == "hybrid" else abi.MODE_COMPACT
+    return m, loom_amd.make_params(mode=m, **params)
+
+
 def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
-         verbose: bool = True, min_active: float = MIN_ACTIVE) -> dict:
-    """Time every candidate of `space` on `inputs` (numpy columns, copied to
-    HBM once) and return the fastest parameters whose active-lane fraction
-    is at least `min_active` (the fastest overall if none reaches it)."""
+         verbose: bool = True, min_active: float = MIN_ACTIVE,
+         compact_space: dict | None = None) -> dict:
+    """Time every candidate of `space` (hybrid) and `compact_space` (compact)
+    on `inputs` (numpy columns, copied to HBM once).  The result holds the
+    fastest hybrid parameters and the fastest form overall, each among the
+    candidates whose active-lane fraction is at least `min_active` (the
+    fastest of all if none reaches it)."""
     import loom_amd
     from loom_amd import abi
 
@@ -60,23 +89,43 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
     din = [loom_amd.DeviceArray.from_numpy(a) for a in inputs]
     dout = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
     space = space or SPACE
+    compact_space = COMPACT_SPACE if compact_space is None else compact_space
     keys = list(space)
+    cands = [("hybrid", {**DEFAULT, **dict(zip(keys, combo))})
+             for combo in itertools.product(*(space[k2] for k2 in keys))]
+    ckeys = list(compact_space)
+    cands += [("compact", dict(zip(ckeys, combo)))
+              for combo in itertools.product(*(compact_space[k2] for k2 in ckeys))]
     results = []
-    for combo in itertools.product(*(space[k2] for k2 in keys)):
-        cand = {**DEFAULT, **dict(zip(keys, combo))}
-        p = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=True, **cand)
-        act = loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p).active_lane_fraction  # warm
+    for mode, cand in cands:
+        m, p = params_of_form(mode, {**cand, "stats": True})
+        act = loom_amd.run(k, m, din, dout, params=p).active_lane_fraction  # warm
+        # timed without counters, the form as it is used; short runs are
+        # repeated until about 20 ms have passed
+        m, p = params_of_form(mode, cand)
         t0 = time.perf_counter()
-        for _ in range(reps):
-            loom_amd.run(k, abi.MODE_HYBRID, din, dout, params=p)
-        ms = (time.perf_counter() - t0) * 1e3 / reps
-        results.append((ms, act, cand))
+        loom_amd.run(k, m, din, dout, params=p)
+        first = (time.perf_counter() - t0) * 1e3
+        count = max(reps, min(50, int(20.0 / max(first, 1e-3))))
+        t0 = time.perf_counter()
+        for _ in range(count):
+            loom_amd.run(k, m, din, dout, params=p)
+        ms = (time.perf_counter() - t0) * 1e3 / count
+        results.append((ms, act, mode, cand))
         if verbose:
-            print(f"tune {kernel_name} {cand} {ms:.3f} ms active {act:.3f}", flush=True)
+            print(f"tune {kernel_name} {mode} {cand} {ms:.3f} ms active {act:.3f}", flush=True)
     results.sort(key=lambda r: r[0])
-    ok = [r for r in results if r[1] >= min_active] or results
-    ms, act, best = ok[0]
+
+    def pick(rs):
+        ok = [r for r in rs if r[1] >= min_active] or rs
+        return ok[0]
+
+    ms, act, _, best = pick([r for r in results if r[2] == "hybrid"])
+    fms, fact, fmode, fbest = pick(results)
+    fparams = {k2: fbest[k2] for k2 in (keys if fmode == "hybrid" else ckeys)}
     return {"params": {k2: best[k2] for k2 in keys}, "ms": ms, "active_lane_fraction": act,
+            "form": {"mode": fmode, "params": fparams, "ms": fms,
+                     "active_lane_fraction": fact},
             "min_active": min_active, "fastest_any_ms": results[0][0], "n": n,
             "candidates": len(results)}
 

@@ -185,6 +185,36 @@ const_as(std::uint64_t bits)
         return std::bit_cast<V>(static_cast<std::uint8_t>(bits));
 }
 
+/// @brief Bits of 1/d for a literal divisor d = ±2^k of the evaluation type
+/// @p E whose reciprocal is a normal number; 0 for any other operand.  For
+/// such a d, x / d and x · (1/d) are the same real number for every x (±0,
+/// subnormals, infinities and NaN included), rounded once, so the quotient
+/// is a multiplication; a correctly rounded f32 division is about eleven
+/// instructions on gfx950.
+template <typename K, std::size_t O, typename E>
+consteval std::uint64_t
+exact_reciprocal_bits()
+{
+    constexpr ir::Operand o = ir::table_of<K>.opnd[O];
+    if constexpr(!std::is_floating_point_v<E> || !o.lit || !std::is_same_v<ctype<o.ty>, E>)
+        return 0;
+    else
+    {
+        constexpr int           mbits = sizeof(E) == 4 ? 23 : 52;
+        constexpr std::uint64_t emax  = sizeof(E) == 4 ? 0xFF : 0x7FF; // all-ones exponent
+        constexpr std::uint64_t b     = sizeof(E) == 4 ? (o.bits & 0xFFFFFFFFull) : o.bits;
+        constexpr std::uint64_t mant  = b & ((std::uint64_t(1) << mbits) - 1);
+        constexpr std::uint64_t e     = (b >> mbits) & emax;
+        constexpr std::uint64_t sign  = b & ~((std::uint64_t(1) << (mbits + (sizeof(E) == 4 ? 8 : 11))) - 1);
+        // d = ±2^(e - bias), bias = (emax - 1) / 2; 1/d has biased exponent
+        // 2·bias - e, a normal exponent when 1 <= e <= emax - 2
+        if constexpr(mant != 0 || e < 1 || e > emax - 2)
+            return 0;
+        else
+            return sign | ((emax - 1 - e) << mbits);
+    }
+}
+
 /// @brief Run-time error flags raised by kernels.
 enum ErrorFlag : std::uint32_t
 {
@@ -290,6 +320,13 @@ exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
             const E a = conv<E>(operand<K, in.a0>(r));
             r.template set<d.slot, D>(conv<D>(arith<op, E>(a)));
         }
+        else if constexpr(op == Op::div && exact_reciprocal_bits<K, in.a0 + 1, E>() != 0)
+        {
+            using U         = std::conditional_t<sizeof(E) == 4, std::uint32_t, std::uint64_t>;
+            constexpr E rcp = std::bit_cast<E>(static_cast<U>(exact_reciprocal_bits<K, in.a0 + 1, E>()));
+            const E     a   = conv<E>(operand<K, in.a0>(r));
+            r.template set<d.slot, D>(conv<D>(a * rcp));
+        }
         else
         {
             const E a = conv<E>(operand<K, in.a0>(r));
@@ -308,6 +345,14 @@ exec_block(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
     [&]<std::size_t... J>(std::index_sequence<J...>)
     { (exec_instr<K, blk.i0 + J>(r, pc, err), ...); }(std::make_index_sequence<blk.ni>{});
     if constexpr(ir::table_of<K>.straightline) pc = HALT;
+}
+
+/// @brief Run block @p B on a lane whose registers are in VGPRs.
+template <typename K, std::size_t B>
+LOOM_DEVICE void
+exec_on(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
+{
+    exec_block<K, B>(r, pc, err);
 }
 
 /// @brief Load kernel inputs of thread @p tid into the register file and

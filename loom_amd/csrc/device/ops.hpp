@@ -134,6 +134,59 @@ fma_(T x, T y, T z)
         return __builtin_fma(x, y, z);
 }
 
+/// @brief Correctly rounded f32 square root, the hardware estimate refined by
+/// its two neighbours: v_sqrt_f32 is within one ulp, and of s - 1 ulp, s and
+/// s + 1 ulp the residuals x - s'·s (exact with fma) pick the nearest root.
+/// The compiler's correctly rounded sqrt does the same on x scaled by 2^32
+/// when x < 2^-96 and returns x itself for ±0, +inf and NaN; for x = ±0,
+/// +inf and every x >= 2^-96 the refinement alone gives the same bits, so
+/// sqrt_rn takes it when no active lane of the wave holds another value
+/// (a wave-uniform branch, no per-lane divergence), else the compiler's.
+LOOM_DEVICE float
+sqrt_refined(float x)
+{
+    const float         s  = __builtin_amdgcn_sqrtf(x);
+    const std::uint32_t si = __builtin_bit_cast(std::uint32_t, s);
+    const float         sm = __builtin_bit_cast(float, si - 1u);
+    const float         sp = __builtin_bit_cast(float, si + 1u);
+    const float         rm = __builtin_fmaf(-sm, s, x);
+    const float         rp = __builtin_fmaf(-sp, s, x);
+    float               r  = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : r;
+}
+
+LOOM_DEVICE float
+sqrt_rn(float x)
+{
+    const bool other = !(x >= 0x1p-96f || x == 0.0f); // NaN, negative, tiny
+    if(__builtin_amdgcn_ballot_w64(other) == 0) return sqrt_refined(x);
+    return __builtin_sqrtf(x);
+}
+
+/// @brief Correctly rounded 1 / (correctly rounded sqrt(x)) in f32.  The
+/// compiler's division is the Newton sequence y = rcp(d), y += y(1 - dy),
+/// q = n·y, q += y(n - dq) twice, wrapped in v_div_scale (a no-op unless an
+/// operand or an intermediate leaves the normal range), v_div_fmas (an fma
+/// when nothing was scaled) and v_div_fixup (the special operands).  For
+/// x in [2^-96, 2^126] the root d lies in [2^-48, 2^63], none of that applies
+/// and the bare sequence gives the same bits; the wave takes it when every
+/// active lane's x is in that range.
+LOOM_DEVICE float
+rsqrt_rn(float x)
+{
+    const bool other = !(x >= 0x1p-96f && x <= 0x1p126f); // also NaN
+    if(__builtin_amdgcn_ballot_w64(other) == 0)
+    {
+        const float d  = sqrt_refined(x);
+        float       y  = __builtin_amdgcn_rcpf(d);
+        y              = __builtin_fmaf(__builtin_fmaf(-d, y, 1.0f), y, y);
+        float       q  = y;
+        q              = __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), y, q);
+        return __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), y, q);
+    }
+    return 1.0f / __builtin_sqrtf(x);
+}
+
 /// @brief Polynomial sine of the reference SIMD backends, valid on [-π, π].
 template <typename T>
 LOOM_DEVICE T
@@ -336,14 +389,14 @@ arith(T a, T b = T{})
         if constexpr(op == Op::rsqrt)
         {
             if constexpr(std::is_same_v<T, float>)
-                return 1.0f / __builtin_sqrtf(a);
+                return rsqrt_rn(a);
             else
                 return 1.0 / __builtin_sqrt(a);
         }
         else if constexpr(op == Op::sqrt)
         {
             if constexpr(std::is_same_v<T, float>)
-                return __builtin_sqrtf(a);
+                return sqrt_rn(a);
             else
                 return __builtin_sqrt(a);
         }

@@ -26,7 +26,7 @@
 /// simd.hpp:2667-2745 without moving dead registers).
 #pragma once
 
-#include "exec.hpp"
+#include "memregs.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -34,6 +34,15 @@
 
 namespace loom::hipvm
 {
+/// @brief Waves per SIMD that the compact and pass kernels of @p K are
+/// compiled to allow at least (amdgpu_waves_per_eu): 1 leaves the register
+/// budget to the compiler.  A kernel whose register file sits just above a
+/// budget step (128 VGPRs for 4 waves) can run faster held under it, a few
+/// values spilled, than at one wave less; the registration of such a kernel
+/// sets its hint (LOOM_HIP_WAVES_HINT).
+template <typename K>
+inline constexpr int waves_hint = 1;
+
 inline constexpr int         kMaxIO    = 32;
 inline constexpr std::size_t kNumStats = 13;
 /// Counter copies: a wave adds into copy (workgroup index mod kStatSlots),
@@ -311,16 +320,16 @@ inline constexpr std::uint32_t kMayHalt = 1u << 16;
 inline constexpr std::uint32_t kIndex   = 0xFFFFu;
 
 /// @brief Run block @p B on the lanes at B and count the work.
-template <typename K, std::size_t B>
+template <typename K, std::size_t B, typename RF>
 LOOM_DEVICE void
-run_block(Regs<K>& r, std::uint32_t& pc, WaveCtx& w)
+run_block(RF& r, std::uint32_t& pc, WaveCtx& w)
 {
     constexpr ir::Block blk = ir::table_of<K>.block[B];
     static_assert(blk.flops <= 2u * blk.ops, "flops exceed twice the op count");
     if constexpr(blk.ops >= kScalarStepOps) w.steps += popc(ballot(pc == B));
     if(pc == B)
     {
-        exec_block<K, B>(r, pc, w.err);
+        exec_on<K, B>(r, pc, w.err);
         if constexpr(blk.ops < kScalarStepOps) w.lane_steps += 1;
         w.lane_ops += blk.ops;
         if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
@@ -351,9 +360,9 @@ inline constexpr std::uint64_t halt_blocks = []
 
 /// @brief Run block @p B on the lanes at B and pick the next block among its
 /// successors (threaded dispatch, simd.hpp:3577-3731).
-template <typename K, std::size_t B>
+template <typename K, std::size_t B, typename RF>
 LOOM_DEVICE std::uint32_t
-step(Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, WaveCtx& w)
+step(RF& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, WaveCtx& w)
 {
     constexpr ir::Block blk = ir::table_of<K>.block[B];
 
@@ -402,9 +411,9 @@ step(Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending, std::uint32_t thr, W
             return NONE;
 
 /// @brief Jump to the step of block @p cur (a wave-uniform value).
-template <typename K>
+template <typename K, typename RF>
 LOOM_DEVICE std::uint32_t
-dispatch(std::uint32_t cur, Regs<K>& r, std::uint32_t& pc, std::uint64_t& pending,
+dispatch(std::uint32_t cur, RF& r, std::uint32_t& pc, std::uint64_t& pending,
          std::uint32_t thr, WaveCtx& w)
 {
     switch(cur)
@@ -475,9 +484,9 @@ major_present(std::uint32_t pc, std::uint32_t& pops)
 /// runs regardless of the thresholds, which guarantees progress for every
 /// claimed batch.  Returns when no block has more than @p thr lanes (thr = 0:
 /// when every lane halted) or when at most @p live_thr lanes remain live.
-template <typename K>
+template <typename K, typename RF>
 LOOM_DEVICE void
-run_to_threshold(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr,
+run_to_threshold(RF& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr,
                  bool force, WaveCtx& w)
 {
     std::uint64_t pending = 0;
@@ -603,9 +612,9 @@ topo_pick(std::uint64_t present)
 
 /// @brief POLICY_TOPO step: run the block of rank @p R, update the present
 /// set from its successors, pick the next rank.
-template <typename K, std::size_t R>
+template <typename K, std::size_t R, typename RF>
 LOOM_DEVICE std::uint32_t
-step_topo(Regs<K>& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
+step_topo(RF& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
 {
     constexpr std::size_t   B   = ir::table_of<K>.order[R];
     constexpr ir::Block     blk = ir::table_of<K>.block[B];
@@ -634,9 +643,9 @@ step_topo(Regs<K>& r, std::uint32_t& pc, std::uint64_t& present, WaveCtx& w)
         else                                                                              \
             return NONE;
 
-template <typename K>
+template <typename K, typename RF>
 LOOM_DEVICE std::uint32_t
-dispatch_topo(std::uint32_t cur, Regs<K>& r, std::uint32_t& pc, std::uint64_t& present,
+dispatch_topo(std::uint32_t cur, RF& r, std::uint32_t& pc, std::uint64_t& present,
               WaveCtx& w)
 {
     switch(cur)
@@ -661,9 +670,9 @@ dispatch_topo(std::uint32_t cur, Regs<K>& r, std::uint32_t& pc, std::uint64_t& p
 
 /// @brief POLICY_TOPO runner: execute until every lane halted or at most
 /// @p live_thr lanes are live (the first step runs regardless with @p force).
-template <typename K>
+template <typename K, typename RF>
 LOOM_DEVICE void
-run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
+run_topo(RF& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
 {
     std::uint64_t present = present_ranks<K>(pc);
     std::uint32_t cur     = topo_pick<K>(present);
@@ -682,9 +691,9 @@ run_topo(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, Wave
 /// @brief POLICY_SWEEP runner: sweeps until every lane halted or, after a
 /// sweep, at most @p live_thr lanes are live (one sweep runs regardless when
 /// @p force is set).
-template <typename K>
+template <typename K, typename RF>
 LOOM_DEVICE void
-run_sweep(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
+run_sweep(RF& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, WaveCtx& w)
 {
     if(!ballot(pc != HALT)) return;
     if(!force && live_thr && popc(ballot(pc != HALT)) <= live_thr) return;
@@ -709,9 +718,9 @@ run_sweep(Regs<K>& r, std::uint32_t& pc, std::uint32_t live_thr, bool force, Wav
 
 /// @brief Run the wave under policy @p P (see run_to_threshold / run_topo /
 /// run_sweep).
-template <typename K, int P>
+template <typename K, int P, typename RF>
 LOOM_DEVICE void
-run_policy(Regs<K>& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr, bool force,
+run_policy(RF& r, std::uint32_t& pc, std::uint32_t thr, std::uint32_t live_thr, bool force,
            WaveCtx& w)
 {
     if constexpr(P == POLICY_TOPO)
@@ -740,7 +749,7 @@ raise_errors(std::uint32_t lane_err, bool over, const CtlArgs& c)
 /// @brief Predicated wave64 kernel: each wave runs 64 consecutive threads to
 /// completion (reference compact_compiler, simd.hpp:3742-4014).
 template <typename K, int P>
-__global__ __launch_bounds__(256) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint<K>))) void
 compact_kernel(IoArgs io, CtlArgs c)
 {
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -755,6 +764,38 @@ compact_kernel(IoArgs io, CtlArgs c)
     WaveCtx w(c);
     run_policy<K, P>(r, pc, 0, 0, true, w);
     if(valid) store_outputs<K>(r, io.out, tid);
+    flush_stats(w);
+    raise_errors(w.err, w.over, c);
+}
+
+/// @brief The register file of a compact run in device memory: column r of
+/// `rows` cells at base + rows * CellLayout<K>::off[r].
+struct MemArgs
+{
+    char*         base = nullptr;
+    std::uint64_t rows = 0;
+};
+
+/// @brief The compact kernel with its register file in device memory: as
+/// compact_kernel, but every block run loads the registers it reads from the
+/// thread's row and stores back those it writes that stay live, so VGPRs hold
+/// one block's working set (sweep policy).
+template <typename K, int P>
+__global__ __launch_bounds__(256) void
+compact_mem_kernel(IoArgs io, CtlArgs c, MemArgs a)
+{
+    const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool          valid = tid < c.n;
+    MemRegs<K>          r{a.base, a.rows, tid};
+    std::uint32_t       pc = HALT;
+    if(valid)
+    {
+        mem_load_inputs<K>(r, io.in, tid);
+        pc = 0;
+    }
+    WaveCtx w(c);
+    run_policy<K, P>(r, pc, 0, 0, true, w);
+    if(valid) mem_store_outputs<K>(r, io.out, tid);
     flush_stats(w);
     raise_errors(w.err, w.over, c);
 }
@@ -1028,7 +1069,7 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 /// pt2_const: 239 -> 138, two -> three waves per SIMD), so runs that do not
 /// regroup launch the RG = false kernel, in which no queue is ever read.
 template <typename K, int P, bool RG>
-__global__ __launch_bounds__(64) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(waves_hint<K>))) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
     HybridCtl* ctl            = h.ctl;

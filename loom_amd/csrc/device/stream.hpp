@@ -71,19 +71,10 @@ struct StreamIo
     std::uint32_t  mode     = SWAP_INIT;
 };
 
-/// @brief Byte offsets of the register columns (per slot, before * S).
+/// @brief Byte offsets of the register columns (per slot, before * S): the
+/// cell layout of a register file in device memory.
 template <typename K>
-struct StateLayout
-{
-    static constexpr auto off = []
-    {
-        std::array<std::uint32_t, ir::max_regs + 1> o{};
-        for(std::size_t r = 0; r < ir::table_of<K>.nr; ++r)
-            o[r + 1] = o[r] + ir::table_of<K>.reg[r].width;
-        return o;
-    }();
-    static constexpr std::uint32_t bytes = off[ir::table_of<K>.nr];
-};
+using StateLayout = CellLayout<K>;
 
 /// @brief Read every register of slot @p slot into this lane.
 template <typename K>

@@ -30,13 +30,16 @@ struct KernelEntry
     /// launchers per reconvergence policy (POLICY_SUCCESSOR_MAJOR, POLICY_TOPO,
     /// POLICY_SWEEP)
     hipError_t (*compact[kNumPolicies])(const IoArgs&, const CtlArgs&, hipStream_t) = {};
+    /// the compact kernel with its register file in device memory (sweep
+    /// policy; MemArgs.rows >= the run's threads)
+    hipError_t (*compact_mem)(const IoArgs&, const CtlArgs&, const MemArgs&, hipStream_t) = nullptr;
     hipError_t (*hybrid_pass[kNumPolicies])(const IoArgs&, const CtlArgs&, const HybridArgs&,
                                  unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
     /// (with or without the regroup paths)
     int (*hybrid_occupancy[kNumPolicies])(bool regroup)                = {};
     /// registers and scratch of the device kernels (0: compact, 1: hybrid)
-    int (*kernel_attrs)(int mode, int policy, bool regroup, int* vgprs, int* scratch) = nullptr;
+    int (*kernel_attrs)(int mode, int policy, int flags, int* vgprs, int* scratch) = nullptr;
     std::string (*dump)() = nullptr;
     /// step trace of threads [0, n) (trace_kernel)
     hipError_t (*trace)(const IoArgs&, std::uint64_t n, std::uint32_t max_steps,
@@ -118,6 +121,17 @@ launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename K>
+hipError_t
+launch_compact_mem(const IoArgs& io, const CtlArgs& c, const MemArgs& m, hipStream_t s)
+{
+    if(c.n == 0) return hipSuccess;
+    const std::uint64_t blocks = (c.n + 255) / 256;
+    hipLaunchKernelGGL((compact_mem_kernel<K, POLICY_SWEEP>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(256), 0, s, io, c, m);
+    return hipGetLastError();
+}
+
 /// Runs that regroup launch the pass kernel with the queue paths compiled
 /// in; the others the kernel without them.
 template <typename K, int P>
@@ -147,16 +161,21 @@ hybrid_occupancy(bool regroup)
 }
 
 /// @brief Registers and scratch of the compiled kernel of @p mode and
-/// @p policy (LOOM_MODE_COMPACT: the predicated kernel; LOOM_MODE_HYBRID: the
-/// pass kernel, with the regroup paths when @p regroup).
+/// @p policy (LOOM_MODE_COMPACT: the predicated kernel, with its register
+/// file in memory under LOOM_RESOURCES_MEMORY; LOOM_MODE_HYBRID: the pass
+/// kernel, with the regroup paths under LOOM_RESOURCES_REGROUP).
 template <typename K>
 int
-kernel_attrs(int mode, int policy, bool regroup, int* vgprs, int* scratch)
+kernel_attrs(int mode, int policy, int flags, int* vgprs, int* scratch)
 {
+    const bool  regroup = (flags & LOOM_RESOURCES_REGROUP) != 0;
+    const bool  memory  = (flags & LOOM_RESOURCES_MEMORY) != 0;
     const void* f = nullptr;
     auto        pick = [&]<int P>()
     {
-        if(mode == LOOM_MODE_COMPACT)
+        if(mode == LOOM_MODE_COMPACT && memory)
+            f = reinterpret_cast<const void*>(compact_mem_kernel<K, POLICY_SWEEP>);
+        else if(mode == LOOM_MODE_COMPACT)
             f = reinterpret_cast<const void*>(compact_kernel<K, P>);
         else if(regroup)
             f = reinterpret_cast<const void*>(hybrid_pass_kernel<K, P, true>);
@@ -263,6 +282,7 @@ make_entry(const char* name)
     e.compact[0]          = &launch_compact<K, POLICY_SUCCESSOR_MAJOR>;
     e.compact[1]          = &launch_compact<K, POLICY_TOPO>;
     e.compact[2]          = &launch_compact<K, POLICY_SWEEP>;
+    e.compact_mem         = &launch_compact_mem<K>;
     e.hybrid_pass[0]      = &launch_hybrid_pass<K, POLICY_SUCCESSOR_MAJOR>;
     e.hybrid_pass[1]      = &launch_hybrid_pass<K, POLICY_TOPO>;
     e.hybrid_pass[2]      = &launch_hybrid_pass<K, POLICY_SWEEP>;
@@ -287,6 +307,13 @@ make_entry(const char* name)
 #define LOOM_HIP_REGISTER_I(LINE, NAME, ...)                              \
     static const bool LOOM_REG_CAT(loom_registered_, LINE) =               \
         ::loom::hipvm::register_kernel(::loom::hipvm::make_entry<__VA_ARGS__>(NAME))
+
+/// @brief Compile the compact and pass kernels of kernel type `...` for at
+/// least @p W waves per SIMD (before its LOOM_HIP_REGISTER, at namespace
+/// scope).
+#define LOOM_HIP_WAVES_HINT(W, ...)                                                   \
+    template <>                                                                        \
+    inline constexpr int ::loom::hipvm::waves_hint<__VA_ARGS__> = (W)
 
 /// @brief Lower kernel type `...` and register it under @p NAME.
 #define LOOM_HIP_REGISTER(NAME, ...) LOOM_HIP_REGISTER_I(__COUNTER__, NAME, __VA_ARGS__)

@@ -117,6 +117,7 @@ context_release(loom_context& x)
     }
     for(void* p: x.hostcol_ptr)
         if(p) (void)hipFree(p);
+    if(x.memregs) (void)hipFree(x.memregs);
     if(x.jobs) (void)hipFree(x.jobs);
     if(x.bctl) (void)hipFree(x.bctl);
     for(auto& t: x.tickets) (void)hipEventDestroy(t.second);
@@ -153,10 +154,31 @@ finish_stats(loom_context& x, loom_run_stats* st, const loom_params* p)
 
 int
 run_compact(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, int policy,
-            loom_run_stats* st)
+            bool memory, loom_run_stats* st)
 {
+    MemArgs m;
+    if(memory)
+    {
+        // one row of register cells per thread, grown on demand
+        const std::size_t need = std::max<std::size_t>(
+            1, static_cast<std::size_t>(c.n) * std::max<std::uint32_t>(e.state_bytes, 4));
+        if(need > x.memregs_bytes)
+        {
+            LOOM_HIP(hipStreamSynchronize(x.stream));
+            if(x.memregs) LOOM_HIP(hipFree(x.memregs));
+            x.memregs       = nullptr;
+            x.memregs_bytes = 0;
+            LOOM_HIP(hipMalloc(&x.memregs, need));
+            x.memregs_bytes = need;
+        }
+        m.base = x.memregs;
+        m.rows = c.n;
+    }
     LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
-    LOOM_HIP(e.compact[policy](io, c, x.stream));
+    if(memory)
+        LOOM_HIP(e.compact_mem(io, c, m, x.stream));
+    else
+        LOOM_HIP(e.compact[policy](io, c, x.stream));
     LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
     LOOM_HIP(hipEventSynchronize(x.ev[1]));
     float ms = 0;
@@ -369,8 +391,13 @@ run_on(loom_context& x, int kernel, int mode, const void* const* inputs, void* c
     LOOM_HIP(hipMemsetAsync(x.err, 0, sizeof(std::uint32_t), x.stream));
     if(c.stats) LOOM_HIP(hipMemsetAsync(x.stats, 0, sizeof(std::uint64_t) * kStatWords, x.stream));
     int r = 0;
+    const bool memory = params && params->register_file == LOOM_REGFILE_MEMORY;
+    if(memory && mode != LOOM_MODE_COMPACT)
+        return fail(LOOM_E_ARG, "a register file in memory runs in the compact form");
+    if(memory && scratch)
+        return fail(LOOM_E_ARG, "a register file in memory needs an allocation; use loom_run");
     if(mode == LOOM_MODE_COMPACT)
-        r = run_compact(x, e, io, c, params ? static_cast<int>(params->reconv) : 0, stats);
+        r = run_compact(x, e, io, c, params ? static_cast<int>(params->reconv) : 0, memory, stats);
     else if(scratch)
         r = run_hybrid_scratch(x, e, io, c, params, stats);
     else
@@ -511,6 +538,8 @@ batch_run_on(loom_context& x, int set, const loom_job* jobs, int njobs, const lo
              loom_run_stats* stats)
 {
     if(int r = validate_params(params)) return r;
+    if(params && params->register_file != LOOM_REGFILE_VGPR)
+        return fail(LOOM_E_ARG, "batched launches keep registers in VGPRs");
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     if(stats) *stats = loom_run_stats{};
@@ -665,6 +694,10 @@ validate_params(const loom_params* params)
         return fail(LOOM_E_ARG, "thresholds must be < 64");
     if(params && params->reconv >= static_cast<std::uint32_t>(kNumPolicies))
         return fail(LOOM_E_ARG, "unknown reconvergence policy");
+    if(params && params->register_file > LOOM_REGFILE_MEMORY)
+        return fail(LOOM_E_ARG, "unknown register file %u", params->register_file);
+    if(params && params->register_file == LOOM_REGFILE_MEMORY && params->reconv != POLICY_SWEEP)
+        return fail(LOOM_E_ARG, "a register file in memory runs under the sweep policy (reconv 2)");
     return 0;
 }
 
@@ -849,10 +882,10 @@ loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch
     if(int r = check_kernel(kernel)) return r;
     if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
         return fail(LOOM_E_ARG, "unknown mode %d", mode);
-    const bool regroup = (policy & LOOM_RESOURCES_REGROUP) != 0;
-    policy &= ~LOOM_RESOURCES_REGROUP;
+    const int flags = policy & (LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY);
+    policy &= ~(LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY);
     if(policy < 0 || policy >= kNumPolicies) return fail(LOOM_E_ARG, "unknown policy %d", policy);
-    if(int r = registry()[kernel].kernel_attrs(mode, policy, regroup, vgprs, scratch_bytes))
+    if(int r = registry()[kernel].kernel_attrs(mode, policy, flags, vgprs, scratch_bytes))
         return fail(r, "hipFuncGetAttributes failed");
     return 0;
 }
@@ -1213,6 +1246,8 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     loom_context& x = *l.x;
     if(!graph) return fail(LOOM_E_ARG, "null graph");
     if(int r = validate_params(params)) return r;
+    if(params && params->register_file != LOOM_REGFILE_VGPR)
+        return fail(LOOM_E_ARG, "batched launches keep registers in VGPRs");
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     // every early return below frees what was made so far (the destructor)

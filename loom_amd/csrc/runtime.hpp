@@ -93,6 +93,10 @@ struct loom_context
     };
     std::vector<Workspace> ws; ///< per kernel index
 
+    /// register file of compact runs in device memory, grown on demand
+    char*       memregs       = nullptr;
+    std::size_t memregs_bytes = 0;
+
     /// device columns of loom_run_host, grown on demand
     std::vector<void*>       hostcol_ptr;
     std::vector<std::size_t> hostcol_bytes;

@@ -145,6 +145,9 @@ loom_stream_create(int kernel, uint64_t slots, const loom_params* params, loom_s
     if(slots == 0 || slots % 64 || slots > (1ull << 31))
         return fail(LOOM_E_ARG, "slots must be a positive multiple of 64, at most 2^31");
     if(int r = validate_params(params)) return r;
+    if(params && params->register_file != LOOM_REGFILE_VGPR)
+        return fail(LOOM_E_ARG, "a stream keeps its registers in device memory between steps and "
+                                "in VGPRs during them");
     auto s    = std::make_unique<loom_stream>();
     s->kernel = kernel;
     s->e      = &kernels()[kernel];

@@ -153,6 +153,12 @@ def resources(name) -> dict:
             abi.check(abi.load().loom_kernel_resources(k.index, mode, pol | flag, C.byref(v),
                                                        C.byref(sc)), "loom_kernel_resources")
             out[f"{mname}/{pname}"] = {"vgprs": v.value, "scratch_bytes": sc.value}
+    # the compact kernel with its register file in device memory (sweep)
+    v, sc = C.c_int(), C.c_int()
+    abi.check(abi.load().loom_kernel_resources(k.index, abi.MODE_COMPACT,
+                                               abi.RECONV_SWEEP | abi.RESOURCES_MEMORY,
+                                               C.byref(v), C.byref(sc)), "loom_kernel_resources")
+    out["compact-memory/sweep"] = {"vgprs": v.value, "scratch_bytes": sc.value}
     return out
 
 

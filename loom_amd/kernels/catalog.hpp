@@ -21,6 +21,10 @@
     X("fp_mix_f64", loom_amd::programs::fp_mix<double>)               \
     X("sincos_f32", loom_amd::programs::sincos<float>)                \
     X("sincos_f64", loom_amd::programs::sincos<double>)               \
+    X("roots_f32", loom_amd::programs::roots<float>)                  \
+    X("roots_f64", loom_amd::programs::roots<double>)                 \
+    X("literal_div_f32", loom_amd::programs::literal_div<float>)      \
+    X("literal_div_f64", loom_amd::programs::literal_div<double>)     \
     X("threefry2x32_u32", loom_amd::programs::threefry2x32<std::uint32_t>) \
     X("cubic_bisect_f32", loom_amd::programs::cubic_bisect<float>)    \
     X("cubic_bisect_f64", loom_amd::programs::cubic_bisect<double>)

@@ -154,6 +154,27 @@ TKERNEL1(sincos, x)
     TCOS(c, x)
 TKERNEL_END2(s, c);
 
+// Square root and reciprocal square root, one of each per input: the
+// correctly rounded results of the op contract over the whole input range
+// (zeros, subnormals, infinities, NaN and negatives included).
+template <std::floating_point T>
+TKERNEL1(roots, x)
+    TSQRT(s, x),
+    TRSQRT(r, x)
+TKERNEL_END2(s, r);
+
+// Division by literals: powers of two (whose quotients the backend forms as
+// products with the exact reciprocal) beside divisors that are not.
+template <std::floating_point T>
+TKERNEL1(literal_div, x)
+    TDIV(q1, x, 0.5),
+    TDIV(q2, x, 16777216),
+    TDIV(q3, x, -0.125),
+    TDIV(q4, x, 3),
+    TDIV(q5, x, 0.1),
+    TDIV(q6, 1, x)
+TKERNEL_END6(q1, q2, q3, q4, q5, q6);
+
 // Float op mix, straight-line, exercising the floating-point contract:
 // arithmetic, fused multiply-add/sub, sqrt/rsqrt, rounding, min/max,
 // copysign, sin/cos polynomials, select, union register and conversions.

@@ -9,13 +9,21 @@ from loom_amd import inspector as inspect
 pytestmark = pytest.mark.gpu
 
 
+# kernels registered with an occupancy hint (LOOM_HIP_WAVES_HINT) and the
+# scratch bytes per lane that holding them to it may cost: the path tracer is
+# held to 128 VGPRs, two values spilled
+HINTED_SCRATCH = {"smallpt9_f32": 16}
+
+
 def test_no_kernel_uses_scratch(gpu):
     print("[gpu] kernel resources", flush=True)
     for k in loom_amd.kernels():
         res = inspect.resources(k.name)
         for form, r in res.items():
             assert 0 < r["vgprs"] <= 512, (k.name, form, r)
-            assert r["scratch_bytes"] == 0, (k.name, form, r)
+            assert r["scratch_bytes"] <= HINTED_SCRATCH.get(k.name, 0), (k.name, form, r)
+            if k.name in HINTED_SCRATCH:
+                assert r["vgprs"] <= 128, (k.name, form, r)
 
 
 def test_inspector_counter_run(gpu):

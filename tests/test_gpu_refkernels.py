@@ -19,6 +19,8 @@ pytestmark = [pytest.mark.gpu,
 FORMS = [
     ("compact_sm", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR)),
     ("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+    ("compact_memory", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
+                                              register_file=abi.REGFILE_MEMORY)),
     ("hybrid_sweep", abi.MODE_HYBRID, {}),
     ("hybrid_sm_regroup", abi.MODE_HYBRID, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR,
                                                 regroup=True, live_threshold=32)),
@@ -69,6 +71,8 @@ def test_reference_kernel_gpu_sized_frame(gpu):
     want, err = O.run(name, [pick], threads=16, path=O.REF_PATH)
     assert err == 0
     forms = [("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+             ("compact_memory", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
+                                                       register_file=abi.REGFILE_MEMORY)),
              ("hybrid_sweep", abi.MODE_HYBRID, {})]
     for fname, mode, params in forms:
         got = loom_amd.run(k, mode, [pick], params=loom_amd.make_params(mode=mode, **params))

@@ -122,3 +122,24 @@ def test_reference_programs_through_the_cpp_facade(gpu):
     print(r.stdout[-2000:], r.stderr[-4000:], flush=True)
     assert r.returncode == 0, r.stderr[-4000:]
     assert "ref_facade_test: ok" in r.stdout
+
+
+NARROW = [n for n in P.NAMES if n.endswith(("_u8", "_i8", "_u16", "_i16"))]
+
+
+@pytest.mark.parametrize("name", NARROW)
+def test_reference_program_stream(gpu, name):
+    """The device-resident stream over the reference's 8- and 16-bit
+    programs: each register's column holds its 32-bit cell, and every output
+    equals the oracle's."""
+    from loom_amd.stream import Stream
+    load_ref_kernels()
+    cols = P.inputs(name, 5000, seed=3)
+    (want,), err = O.run(name, cols, threads=16, path=O.REF_PATH)
+    assert err == 0
+    print(f"[gpu] stream {name}", flush=True)
+    s = Stream(name, 256, live_threshold=16)
+    (got,) = s.run_all(cols)
+    s.close()
+    assert got.dtype == want.dtype
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), name
