@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-regroup", action="store_true", help="skip the queue-regrouping form")
     ap.add_argument("--no-int", action="store_true", help="skip the fib and Collatz configs")
+    ap.add_argument("--no-ref-pt2", action="store_true",
+                    help="skip the reference's pt2_const path tracer (1024x1024) in every "
+                         "form (built by oracle/build_ref.py into "
+                         "oracle/_ref/libloom_refkernels.so)")
     ap.add_argument("--qps", type=float, default=SERVE_QPS, help="serving arrival rate")
     ap.add_argument("--requests", type=int, default=SERVE_REQUESTS)
     return ap.parse_args()
@@ -146,6 +150,53 @@ def serving(qps, count):
                  "workload": "Poisson arrivals; half frames 128^2-512^2 at 4-16 spp, half "
                              "fib/Collatz of 2^14-2^20 threads"})
     return summ
+
+
+def ref_pt2_leg():
+    """The reference's own path tracer, pt2_const (blur 5, threshold 0.0005,
+    depth 20) on a 1024x1024 frame, compiled unmodified from its checkout by
+    this backend (oracle/build_ref.py): frame time, rays/s and active lanes
+    of the compact sweep form, the hybrid form (refill, tuned live
+    threshold), the compact form with its register file in memory, and the
+    queue-regrouping form under the reference's successor-major policy at two
+    block-population thresholds (regrouping passes: a wave stops when its
+    best block has no more lanes than the threshold and its threads are
+    queued by label for the next pass)."""
+    import ctypes as C
+    import loom_amd
+    from loom_amd import abi
+    lib = ROOT / "oracle" / "_ref" / "libloom_refkernels.so"
+    if not lib.exists():
+        return None
+    C.CDLL(str(lib))
+    k = loom_amd.kernel("ref_pt2_const_b5_t0.0005_d20_1024")
+    # ppp = x << 16 | y over the frame (reference smallpt/vm/enc.hpp:30-34)
+    ppp = np.array([(x << 16) | y for x in range(1024) for y in range(1024)], np.uint32)
+    dev = [loom_amd.DeviceArray.from_numpy(ppp)]
+    forms = {
+        "compact_sweep": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
+        "hybrid_live56": (abi.MODE_HYBRID, dict(live_threshold=56)),
+        "compact_memory_registers": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
+                                                            register_file=abi.REGFILE_MEMORY)),
+        "regroup_successor_major_thr8": (abi.MODE_HYBRID, dict(
+            reconv=abi.RECONV_SUCCESSOR_MAJOR, regroup=True, threshold=8, live_threshold=0)),
+        "regroup_successor_major_thr32": (abi.MODE_HYBRID, dict(
+            reconv=abi.RECONV_SUCCESSOR_MAJOR, regroup=True, threshold=32, live_threshold=0)),
+    }
+    out, first = {}, None
+    for name, (mode, params) in forms.items():
+        outs = [loom_amd.DeviceArray(len(ppp), dt) for dt in k.outputs]
+        r = loom_amd.run(k, mode, dev, outs, params=loom_amd.make_params(mode=mode, **params))
+        got = [o.numpy() for o in outs]
+        if first is None:
+            first = got
+        rays = int(got[3].astype(np.uint64).sum())
+        out[name] = {"ms": r.ms_total, "rays_per_s": rays / (r.ms_total / 1e3),
+                     "active_lane_fraction": r.active_lane_fraction, "passes": r.passes,
+                     "outputs_identical": all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                                              for x, y in zip(got, first))}
+        print(f"ref pt2_const 1024x1024 {name}: {r.ms_total:.1f} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def regroup_leg(k, ins, outs, h_img):
@@ -400,6 +451,11 @@ def main():
         result["regroup"] = regroup_leg(k, ins, outs, h_img)
     if not a.no_int:
         result["integer_configs"] = integer_configs(a.steps)
+    if not a.no_ref_pt2:
+        try:
+            result["ref_pt2_const_1024"] = ref_pt2_leg()
+        except Exception as e:  # the reference's kernels are optional here
+            result["ref_pt2_const_1024"] = {"error": str(e)}
     if not a.no_batch:
         result["batch"] = mixed_batch(max(1, min(a.steps, 3)))
     if not a.no_serve:

@@ -375,11 +375,13 @@ int loom_graph_destroy(loom_graph* graph);
  * new inputs into a stream as old ones halt (src/vm/mp.hpp:61-131, 316-385).
  * A request is admitted FIFO when its device columns fit the HBM budget and
  * a job slot is free; every step is ONE batch launch that advances all
- * resident requests by a slice of their threads (step_threads shared
- * among them), so requests that arrive while others run join the next step
- * and share the device with them.  A request completes at the end of the
- * step that ran its last slice; its outputs are copied back and its device
- * memory is released. */
+ * resident requests of the batch set's kernels by a slice of their threads
+ * (step_threads shared among them), so requests that arrive while others
+ * run join the next step and share the device with them.  A request of any
+ * other registered kernel runs its slice of the step as a hybrid launch of
+ * its own, after the batch launch, on the same stream.  A request completes
+ * at the end of the step that ran its last slice; its outputs are copied
+ * back and its device memory is released. */
 typedef struct loom_server loom_server;
 
 typedef struct loom_server_config
@@ -388,12 +390,13 @@ typedef struct loom_server_config
     uint32_t    max_resident; /* requests resident at once, 1..LOOM_MAX_JOBS */
     uint64_t    hbm_budget;   /* bytes of device columns of resident requests */
     uint64_t    step_threads; /* threads advanced per step, shared by residents */
-    loom_params params;       /* batch launch parameters */
+    loom_params params;       /* parameters of the batch launches and of the
+                                 launches of kernels outside the set */
 } loom_server_config;
 
 typedef struct loom_request
 {
-    int32_t     kernel;                /* global kernel index, member of the set */
+    int32_t     kernel;                /* global kernel index (any registered kernel) */
     uint32_t    broadcast;             /* bit i: input i is scalars[i], not a column */
     uint64_t    n;                     /* threads */
     const void* inputs[LOOM_MAX_IO];   /* host columns of n elements */

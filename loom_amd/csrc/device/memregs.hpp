@@ -45,13 +45,16 @@ inline constexpr auto live_out = []
     return o;
 }();
 
-/// @brief This lane's row of a register file in device memory.
+/// @brief This lane's row of a register file in device memory.  A column's
+/// address is wave-uniform (scalar registers) and the lane's offset in it is
+/// its row times the cell size, so an access is one load or store with a
+/// scalar base and a 32-bit lane offset.
 template <typename K>
 struct MemRegs
 {
-    char*         base = nullptr; ///< column 0
-    std::uint64_t rows = 0;       ///< elements per column
-    std::uint64_t row  = 0;       ///< this lane's row
+    char*         base = nullptr; ///< column 0 (wave-uniform)
+    std::uint64_t rows = 0;       ///< elements per column (wave-uniform)
+    std::uint32_t row  = 0;       ///< this lane's row
 
     template <std::size_t R>
     LOOM_DEVICE std::uint64_t
@@ -76,12 +79,24 @@ struct MemRegs
     }
 };
 
+/// @brief Hide a wave-uniform pointer from the optimiser, so the column
+/// addresses built on it are formed where they are used.  Without it the
+/// compiler hoists every column's address out of the scheduling loop and
+/// keeps them all live, as 64-bit per-lane values, for the whole kernel.
+LOOM_DEVICE void
+opaque_base(char*& p)
+{
+    asm volatile("" : "+s"(p));
+}
+
 /// @brief Run block @p B on a lane whose registers are in device memory.
 template <typename K, std::size_t B>
 LOOM_DEVICE void
-exec_on(MemRegs<K>& m, std::uint32_t& pc, std::uint32_t& err)
+exec_on(const MemRegs<K>& mr, std::uint32_t& pc, std::uint32_t& err)
 {
     constexpr ir::Block blk = ir::table_of<K>.block[B];
+    MemRegs<K>          m   = mr;
+    opaque_base(m.base);
     Regs<K>             r;
     [&]<std::size_t... R>(std::index_sequence<R...>)
     {

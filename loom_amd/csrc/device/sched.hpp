@@ -34,12 +34,14 @@
 
 namespace loom::hipvm
 {
-/// @brief Waves per SIMD that the compact and pass kernels of @p K are
-/// compiled to allow at least (amdgpu_waves_per_eu): 1 leaves the register
-/// budget to the compiler.  A kernel whose register file sits just above a
-/// budget step (128 VGPRs for 4 waves) can run faster held under it, a few
-/// values spilled, than at one wave less; the registration of such a kernel
-/// sets its hint (LOOM_HIP_WAVES_HINT).
+/// @brief Waves per SIMD that the hybrid pass kernels of @p K are compiled
+/// to allow at least (amdgpu_waves_per_eu): 1 leaves the register budget to
+/// the compiler.  A kernel whose register file sits just above a budget step
+/// (128 VGPRs for 4 waves) can run faster held under it, a few values
+/// spilled, than at one wave less; the registration of such a kernel sets
+/// its hint (LOOM_HIP_WAVES_HINT).  The compact kernels are left to the
+/// compiler: under the successor-major policy they can need far more
+/// registers, which a hint would spill.
 template <typename K>
 inline constexpr int waves_hint = 1;
 
@@ -749,7 +751,7 @@ raise_errors(std::uint32_t lane_err, bool over, const CtlArgs& c)
 /// @brief Predicated wave64 kernel: each wave runs 64 consecutive threads to
 /// completion (reference compact_compiler, simd.hpp:3742-4014).
 template <typename K, int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint<K>))) void
+__global__ __launch_bounds__(256) void
 compact_kernel(IoArgs io, CtlArgs c)
 {
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -786,7 +788,7 @@ compact_mem_kernel(IoArgs io, CtlArgs c, MemArgs a)
 {
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool          valid = tid < c.n;
-    MemRegs<K>          r{a.base, a.rows, tid};
+    MemRegs<K>          r{a.base, a.rows, static_cast<std::uint32_t>(tid)};
     std::uint32_t       pc = HALT;
     if(valid)
     {

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <set>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -234,14 +235,17 @@ serve(loom_server* s, Source& src, clock_type::time_point t0, loom_server_stats*
     loom_server_stats st{};
     auto now = [&] { return std::chrono::duration<double, std::milli>(clock_type::now() - t0).count(); };
 
-    // kernel infos of the members
+    // kernel infos: every registered kernel can be served; members of the
+    // batch set run together in one batch launch per step, any other kernel
+    // runs its slice of the step as a launch of its own
     std::map<int, loom_kernel_info> info;
-    for(int k: s->members)
+    for(int k = 0, nk = loom_kernel_count(); k < nk; ++k)
     {
         loom_kernel_info ki{};
         if(int r = loom_kernel_info_get(k, &ki)) return r;
         info.emplace(k, ki);
     }
+    const std::set<int> in_set(s->members.begin(), s->members.end());
 
     // A request moves through: pending (arrived, waiting for a slot and
     // memory) -> loading (inputs copying in on the copy stream) -> running
@@ -457,31 +461,61 @@ serve(loom_server* s, Source& src, clock_type::time_point t0, loom_server_stats*
         // ---- one step: a slice of every running request in one batch launch ----
         const std::uint64_t quota =
             std::max<std::uint64_t>(64, (s->cfg.step_threads / running.size() + 63) / 64 * 64);
-        jobs.assign(running.size(), loom_job{});
+        jobs.clear();
+        std::vector<std::uint64_t> ends(running.size());
         for(std::size_t i = 0; i < running.size(); ++i)
         {
             const loom_request& q = *running[i].req;
-            loom_job&           j = jobs[i];
-            j.kernel              = q.kernel;
-            j.begin               = running[i].cursor;
-            j.end                 = std::min<std::uint64_t>(q.n, running[i].cursor + quota);
+            ends[i] = std::min<std::uint64_t>(q.n, running[i].cursor + quota);
+            if(!in_set.count(q.kernel)) continue;
+            loom_job j{};
+            j.kernel = q.kernel;
+            j.begin  = running[i].cursor;
+            j.end    = ends[i];
             for(std::size_t k = 0; k < running[i].in.size(); ++k) j.inputs[k] = running[i].in[k];
             for(std::size_t k = 0; k < running[i].out.size(); ++k) j.outputs[k] = running[i].out[k];
+            jobs.push_back(j);
         }
         loom_run_stats rs{};
-        if(int e = loom_ctx_batch_run(ctx, s->cfg.batch_set, jobs.data(),
-                                      static_cast<int>(jobs.size()), &s->cfg.params, &rs))
+        if(!jobs.empty())
         {
-            for(auto& r: running) r.req->status = e;
-            return fail(e);
+            if(int e = loom_ctx_batch_run(ctx, s->cfg.batch_set, jobs.data(),
+                                          static_cast<int>(jobs.size()), &s->cfg.params, &rs))
+            {
+                for(auto& r: running) r.req->status = e;
+                return fail(e);
+            }
+            st.gpu_ms += rs.ms_main;
         }
-        st.gpu_ms += rs.ms_main;
+        // kernels outside the batch set: their slice of the step, one launch
+        // each, on the same stream (columns offset to the slice's first thread)
+        for(std::size_t i = 0; i < running.size(); ++i)
+        {
+            const loom_request& q = *running[i].req;
+            if(in_set.count(q.kernel)) continue;
+            const loom_kernel_info& ki = info.at(q.kernel);
+            const std::uint64_t     b  = running[i].cursor;
+            const void*             ins[LOOM_MAX_IO]{};
+            void*                   outs[LOOM_MAX_IO]{};
+            for(int k = 0; k < ki.n_inputs; ++k)
+                ins[k] = running[i].in[k] + b * type_bytes(ki.input_type[k]);
+            for(int k = 0; k < ki.n_outputs; ++k)
+                outs[k] = running[i].out[k] + b * type_bytes(ki.output_type[k]);
+            loom_run_stats ks{};
+            if(int e = loom_ctx_run(ctx, q.kernel, LOOM_MODE_HYBRID, ins, outs, ends[i] - b,
+                                    &s->cfg.params, &ks))
+            {
+                for(auto& r: running) r.req->status = e;
+                return fail(e);
+            }
+            st.gpu_ms += ks.ms_main;
+        }
         ++st.steps;
         std::vector<Resident> keep;
         keep.reserve(running.size());
         for(std::size_t i = 0; i < running.size(); ++i)
         {
-            running[i].cursor = jobs[i].end;
+            running[i].cursor = ends[i];
             ++running[i].req->steps;
             if(running[i].cursor >= running[i].req->n)
             {

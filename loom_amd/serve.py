@@ -57,6 +57,7 @@ class RequestSpec:
     h: int = 0
     spp: int = 0
     offset: int = 0  # fib/Collatz: first element of the shared input pool
+    inputs: list | None = None  # host columns of any other kernel's request
 
 
 def poisson_trace(qps: float, count: int, seed: int = workloads.SEED + 3,
@@ -108,9 +109,11 @@ class Pools:
         return p
 
     def inputs_of(self, spec: RequestSpec, full: bool = False):
-        """Host input columns of a request (views of the pools).  A frame's
-        second input is one word broadcast to every thread; `full` spells
-        it out as a column."""
+        """Host input columns of a request (views of the pools, or the
+        request's own columns).  A frame's second input is one word
+        broadcast to every thread; `full` spells it out as a column."""
+        if spec.inputs is not None:
+            return spec.inputs
         if spec.kernel.startswith("smallpt9"):
             job = workloads.pt_job(spec.w, spec.h, spec.spp)
             return [self.sid.array[:spec.n],
@@ -154,13 +157,14 @@ class Server:
             q.n = s.n
             q.arrival_ms = s.arrival_ms
             ins = pools.inputs_of(s)
-            if s.kernel.startswith("smallpt9"):
+            if s.inputs is None and s.kernel.startswith("smallpt9"):
                 # the frame word is the same for every thread: broadcast it
                 q.inputs[0] = ins[0].ctypes.data
                 q.broadcast = 1 << 1
                 q.scalars[1] = workloads.pt_job(s.w, s.h, s.spp)
             else:
-                q.inputs[0] = ins[0].ctypes.data
+                for j, col in enumerate(ins):
+                    q.inputs[j] = col.ctypes.data
             keep.append(ins)
             if outputs is None:
                 outs.append([np.zeros(s.n, dt) for dt in k.outputs])

@@ -6,5 +6,9 @@
 
 #include "ref_kernels.hpp"
 
+// The pass kernel needs 138 VGPRs left to itself (three waves per SIMD);
+// held to 128 it spills nothing and runs four waves per SIMD.
+LOOM_HIP_WAVES_HINT(4, loom::vm::pt2_const<float, loom_ref::c_b5_t0005_d20>);
+
 #define LOOM_REGISTER_ONE(NAME, ...) LOOM_HIP_REGISTER(NAME, __VA_ARGS__);
 LOOM_REF_CATALOG_PT2_CONST(LOOM_REGISTER_ONE)

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import loom_amd
-from loom_amd import abi, serve
+from loom_amd import abi, serve, workloads
 from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -60,3 +60,33 @@ def test_budget_rejects_oversized_request(pools):
     for i in (0, 2):
         want, _ = O.run(specs[i].kernel, pools.inputs_of(specs[i]))
         assert np.array_equal(outs[i][0], want[0])
+
+
+def test_kernels_outside_the_batch_set(pools):
+    """Requests of kernels outside the server's batch set run their slice of
+    each step as launches of their own, interleaved with the batch launches
+    of the others; every result equals its run alone."""
+    base = serve.poisson_trace(qps=4000.0, count=24, scale=4, seed=12)
+    extra = []
+    for i, (name, n) in enumerate((("cubic_bisect_f32", 70000), ("threefry2x32_u32", 50000),
+                                   ("cubic_bisect_f64", 30000), ("collatz_u64", 9000))):
+        if name.startswith("cubic"):
+            ins = workloads.cubic_inputs(n, np.float32 if name.endswith("f32") else np.float64)
+        elif name.startswith("threefry"):
+            k = loom_amd.kernel(name)
+            rng = np.random.default_rng(70 + i)
+            ins = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(dt) for dt in k.inputs]
+        else:
+            ins = [workloads.collatz_inputs(n, seed=71)]
+        extra.append(serve.RequestSpec(name, n, base[2 * i].arrival_ms + 0.01, inputs=ins))
+    specs = sorted(base + extra, key=lambda s: s.arrival_ms)
+    srv = serve.Server(hbm_budget=96 << 20, max_resident=8, step_threads=1 << 16)
+    print(f"[gpu] serve {len(specs)} requests, 3 outside the batch set", flush=True)
+    reqs, st, outs = srv.run(specs, pools, realtime=True)
+    srv.close()
+    assert all(q.status == 0 for q in reqs)
+    assert max(q.steps for q in reqs) > 1
+    for s, q, o in zip(specs, reqs, outs):
+        alone = loom_amd.run(s.kernel, abi.MODE_HYBRID, pools.inputs_of(s, full=True)).outputs
+        for g, a in zip(o, alone):
+            assert np.array_equal(bits(g), bits(a)), (s.kernel, s.n)
