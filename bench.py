@@ -110,6 +110,7 @@ def mixed_batch(steps):
 
 
 SERVE_CHECKED = 24  # requests per serving run whose outputs are compared
+WGR_PHASE = 128     # dispatches between two workgroup sorts (reference path tracer)
 
 
 def serving(qps, count):
@@ -157,11 +158,13 @@ def ref_pt2_leg():
     depth 20) on a 1024x1024 frame, compiled unmodified from its checkout by
     this backend (oracle/build_ref.py): frame time, rays/s and active lanes
     of the compact sweep form, the hybrid form (refill, tuned live
-    threshold), the compact form with its register file in memory, and the
-    queue-regrouping form under the reference's successor-major policy at two
-    block-population thresholds (regrouping passes: a wave stops when its
-    best block has no more lanes than the threshold and its threads are
-    queued by label for the next pass)."""
+    threshold), the hybrid form regrouping the threads of each workgroup by
+    block through LDS every WGR_PHASE dispatches, the compact form with its
+    register file in memory, and the queue-regrouping form under the
+    reference's successor-major policy at two block-population thresholds
+    (regrouping passes: a wave stops when its best block has no more lanes
+    than the threshold and its threads are queued by label for the next
+    pass)."""
     import ctypes as C
     import loom_amd
     from loom_amd import abi
@@ -176,6 +179,8 @@ def ref_pt2_leg():
     forms = {
         "compact_sweep": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
         "hybrid_live56": (abi.MODE_HYBRID, dict(live_threshold=56)),
+        "hybrid_workgroup_regroup": (abi.MODE_HYBRID, dict(
+            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE)),
         "compact_memory_registers": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
                                                             register_file=abi.REGFILE_MEMORY)),
         "regroup_successor_major_thr8": (abi.MODE_HYBRID, dict(

@@ -98,7 +98,11 @@ typedef struct loom_params
                                 (reference UTR, simd.hpp:3270-3281) */
     uint32_t live_threshold; /* hybrid: stop when live lanes <= this (0..63) */
     uint32_t refill;         /* hybrid: admit new threads into free lanes */
-    uint32_t regroup;        /* hybrid: spill survivors to per-label queues */
+    uint32_t regroup;        /* hybrid: LOOM_REGROUP_NONE, LOOM_REGROUP_QUEUES
+                                (spill survivors to per-label queues for the
+                                next pass) or LOOM_REGROUP_WORKGROUP (sort the
+                                threads of a workgroup by block through LDS
+                                between run phases; reconv 0 or 2) */
     uint32_t reconv;         /* reconvergence policy: 0 = successor-major
                                 (reference simd.hpp:3974-4005), 1 = topological
                                 (loop heads wait for their bodies), 2 = sweep
@@ -118,9 +122,14 @@ typedef struct loom_params
                                 loads what it reads and stores what stays
                                 live, so VGPRs hold one block's working set;
                                 sweep policy only) */
+    uint32_t regroup_phase;  /* LOOM_REGROUP_WORKGROUP: dispatches of a run
+                                phase between two sorts (0: 64) */
 } loom_params;
 #define LOOM_REGFILE_VGPR 0
 #define LOOM_REGFILE_MEMORY 1
+#define LOOM_REGROUP_NONE 0
+#define LOOM_REGROUP_QUEUES 1
+#define LOOM_REGROUP_WORKGROUP 2
 /* A NULL loom_params* means: threshold 0, refill on, regroup off, no stats,
  * successor-major for compact, sweep with live_threshold 48 for hybrid and
  * batched launches. */
@@ -161,6 +170,7 @@ int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
  * for the compact kernel whose register file is in device memory). */
 #define LOOM_RESOURCES_REGROUP 0x100
 #define LOOM_RESOURCES_MEMORY 0x200
+#define LOOM_RESOURCES_WORKGROUP 0x400 /* the workgroup-regrouping kernel (reconv 0 or 2) */
 int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
 /* Register r (0 <= r < n_registers) of a kernel: its name (NUL-terminated,
  * at most len bytes), first type (loom_type), width in bytes, number of

@@ -167,12 +167,15 @@ class RunResult:
 
 def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, waves_per_cu=0,
                 queue_capacity=0, max_steps=0, stats=True, reconv=None,
-                mode=abi.MODE_COMPACT, register_file=abi.REGFILE_VGPR) -> abi.Params:
+                mode=abi.MODE_COMPACT, register_file=abi.REGFILE_VGPR,
+                regroup_phase=0) -> abi.Params:
     """Run parameters; `reconv` defaults to successor-major for the compact
     form (the reference policy) and to the sweep policy for hybrid,
     `live_threshold` to 48 for hybrid (0 for compact, which does not
     refill); `register_file` = REGFILE_MEMORY keeps a compact run's
-    registers in device memory (sweep policy)."""
+    registers in device memory (sweep policy); `regroup` = REGROUP_WORKGROUP
+    sorts a workgroup's threads by block between run phases of
+    `regroup_phase` dispatches."""
     p = abi.Params()
     if reconv is None:
         reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_SWEEP
@@ -182,12 +185,13 @@ def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, wa
     p.threshold = int(threshold)
     p.live_threshold = int(live_threshold)
     p.refill = 1 if refill else 0
-    p.regroup = 1 if regroup else 0
+    p.regroup = int(regroup)  # False/True, or abi.REGROUP_*
     p.waves_per_cu = int(waves_per_cu)
     p.queue_capacity = int(queue_capacity)
     p.max_steps = int(max_steps)
     p.collect_stats = 1 if stats else 0
     p.register_file = int(register_file)
+    p.regroup_phase = int(regroup_phase)
     return p
 
 

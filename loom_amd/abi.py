@@ -20,6 +20,10 @@ MODE_HYBRID = 1
 # or'ed into loom_kernel_resources' policy: the pass kernel that regroups
 RESOURCES_REGROUP = 0x100
 RESOURCES_MEMORY = 0x200
+RESOURCES_WORKGROUP = 0x400
+REGROUP_NONE = 0
+REGROUP_QUEUES = 1
+REGROUP_WORKGROUP = 2
 REGFILE_VGPR = 0
 REGFILE_MEMORY = 1
 
@@ -79,6 +83,7 @@ class Params(C.Structure):
         ("queue_capacity", C.c_uint64),
         ("max_steps", C.c_uint64),
         ("register_file", C.c_uint32),
+        ("regroup_phase", C.c_uint32),
     ]
 
 

@@ -1,8 +1,10 @@
-"""Autotuner: picks the execution form of a kernel (the hybrid form or the
-predicated compact form) and its settings by timing candidates on a sample
-of the real workload.  Hybrid candidates vary the live-lane refill
-threshold, the regroup switch, the reconvergence policy and the persistent
-grid (waves per CU); compact candidates vary the reconvergence policy.
+"""Autotuner: picks the execution form of a kernel (the hybrid form, the
+hybrid form regrouping within workgroups, or the predicated compact form)
+and its settings by timing candidates on a sample of the real workload.
+Hybrid candidates vary the live-lane refill threshold, the regroup switch,
+the reconvergence policy and the persistent grid (waves per CU);
+workgroup-regrouping candidates the phase length; compact candidates the
+reconvergence policy.
 
     python -m loom_amd.autotune smallpt9_f32      # tunes, writes tuned.json
     python -m loom_amd.autotune batch:pt_int_mix  # the 64-job mixed batch
@@ -42,6 +44,10 @@ SPACE = {
 # the compact form: one wave per 64 threads, reconvergence policy only
 COMPACT_SPACE = {"reconv": [0, 1, 2]}
 
+# the hybrid form regrouping within workgroups (sweep policy): the phase
+# length between two sorts
+WGR_SPACE = {"regroup_phase": [64, 256, 1024]}
+
 
 def best_params(kernel_name: str) -> dict:
     """Tuned hybrid parameters of `kernel_name` (defaults if never tuned)."""
@@ -74,11 +80,13 @@ def params_of_form(mode: str, params: dict):
 
 def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
          verbose: bool = True, min_active: float = MIN_ACTIVE,
-         compact_space: dict | None = None) -> dict:
-    """Time every candidate of `space` (hybrid) and `compact_space` (compact)
-    on `inputs` (numpy columns, copied to HBM once).  The result holds the
-    fastest hybrid parameters and the fastest form overall, each among the
-    candidates whose active-lane fraction is at least `min_active` (the
+         compact_space: dict | None = None, wgr_space: dict | None = None) -> dict:
+    """Time every candidate of `space` (hybrid), `wgr_space` (hybrid,
+    regrouping within workgroups) and `compact_space` (compact) on `inputs`
+    (numpy columns, copied to HBM once).  The result holds the fastest
+    hybrid parameters without workgroup regrouping (what batched launches
+    and the serving loop can use) and the fastest form overall, each among
+    the candidates whose active-lane fraction is at least `min_active` (the
     fastest of all if none reaches it)."""
     import loom_amd
     from loom_amd import abi
@@ -96,6 +104,10 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
     ckeys = list(compact_space)
     cands += [("compact", dict(zip(ckeys, combo)))
               for combo in itertools.product(*(compact_space[k2] for k2 in ckeys))]
+    wgr_space = WGR_SPACE if wgr_space is None else wgr_space
+    wkeys = list(wgr_space)
+    cands += [("hybrid", {**DEFAULT, "regroup": 2, "reconv": 2, **dict(zip(wkeys, combo))})
+              for combo in itertools.product(*(wgr_space[k2] for k2 in wkeys))]
     results = []
     for mode, cand in cands:
         m, p = params_of_form(mode, {**cand, "stats": True})
@@ -120,9 +132,14 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
         ok = [r for r in rs if r[1] >= min_active] or rs
         return ok[0]
 
-    ms, act, _, best = pick([r for r in results if r[2] == "hybrid"])
+    ms, act, _, best = pick([r for r in results if r[2] == "hybrid" and r[3]["regroup"] != 2])
     fms, fact, fmode, fbest = pick(results)
-    fparams = {k2: fbest[k2] for k2 in (keys if fmode == "hybrid" else ckeys)}
+    if fmode == "compact":
+        fparams = {k2: fbest[k2] for k2 in ckeys}
+    elif fbest["regroup"] == 2:
+        fparams = {"regroup": 2, "reconv": 2, **{k2: fbest[k2] for k2 in wkeys}}
+    else:
+        fparams = {k2: fbest[k2] for k2 in keys}
     return {"params": {k2: best[k2] for k2 in keys}, "ms": ms, "active_lane_fraction": act,
             "form": {"mode": fmode, "params": fparams, "ms": fms,
                      "active_lane_fraction": fact},

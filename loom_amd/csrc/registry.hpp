@@ -11,6 +11,7 @@
 #include "device/batch.hpp"
 #include "device/stream.hpp"
 #include "device/trace.hpp"
+#include "device/wgr.hpp"
 
 #include <loom_amd.h>
 
@@ -33,6 +34,11 @@ struct KernelEntry
     /// the compact kernel with its register file in device memory (sweep
     /// policy; MemArgs.rows >= the run's threads)
     hipError_t (*compact_mem)(const IoArgs&, const CtlArgs&, const MemArgs&, hipStream_t) = nullptr;
+    /// the workgroup-regrouping kernel (successor-major, sweep) and its
+    /// resident workgroups per CU
+    hipError_t (*wgr[2])(const IoArgs&, const CtlArgs&, const WgrArgs&, unsigned grid,
+                         hipStream_t) = {};
+    int (*wgr_occupancy[2])() = {};
     hipError_t (*hybrid_pass[kNumPolicies])(const IoArgs&, const CtlArgs&, const HybridArgs&,
                                  unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
@@ -132,6 +138,26 @@ launch_compact_mem(const IoArgs& io, const CtlArgs& c, const MemArgs& m, hipStre
     return hipGetLastError();
 }
 
+template <typename K, int P>
+hipError_t
+launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, unsigned grid, hipStream_t s)
+{
+    if(c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL((wgr_kernel<K, P>), dim3(grid), dim3(64 * kWgrWaves), 0, s, io, c, g);
+    return hipGetLastError();
+}
+
+template <typename K, int P>
+int
+wgr_occupancy()
+{
+    int nblk = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, wgr_kernel<K, P>, 64 * kWgrWaves, 0) !=
+       hipSuccess)
+        return 1;
+    return nblk > 0 ? nblk : 1;
+}
+
 /// Runs that regroup launch the pass kernel with the queue paths compiled
 /// in; the others the kernel without them.
 template <typename K, int P>
@@ -170,10 +196,15 @@ kernel_attrs(int mode, int policy, int flags, int* vgprs, int* scratch)
 {
     const bool  regroup = (flags & LOOM_RESOURCES_REGROUP) != 0;
     const bool  memory  = (flags & LOOM_RESOURCES_MEMORY) != 0;
+    const bool  wgr     = (flags & LOOM_RESOURCES_WORKGROUP) != 0;
     const void* f = nullptr;
     auto        pick = [&]<int P>()
     {
-        if(mode == LOOM_MODE_COMPACT && memory)
+        if(mode == LOOM_MODE_HYBRID && wgr)
+            f = P == POLICY_SUCCESSOR_MAJOR ?
+                    reinterpret_cast<const void*>(wgr_kernel<K, POLICY_SUCCESSOR_MAJOR>) :
+                    reinterpret_cast<const void*>(wgr_kernel<K, POLICY_SWEEP>);
+        else if(mode == LOOM_MODE_COMPACT && memory)
             f = reinterpret_cast<const void*>(compact_mem_kernel<K, POLICY_SWEEP>);
         else if(mode == LOOM_MODE_COMPACT)
             f = reinterpret_cast<const void*>(compact_kernel<K, P>);
@@ -283,6 +314,10 @@ make_entry(const char* name)
     e.compact[1]          = &launch_compact<K, POLICY_TOPO>;
     e.compact[2]          = &launch_compact<K, POLICY_SWEEP>;
     e.compact_mem         = &launch_compact_mem<K>;
+    e.wgr[0]              = &launch_wgr<K, POLICY_SUCCESSOR_MAJOR>;
+    e.wgr[1]              = &launch_wgr<K, POLICY_SWEEP>;
+    e.wgr_occupancy[0]    = &wgr_occupancy<K, POLICY_SUCCESSOR_MAJOR>;
+    e.wgr_occupancy[1]    = &wgr_occupancy<K, POLICY_SWEEP>;
     e.hybrid_pass[0]      = &launch_hybrid_pass<K, POLICY_SUCCESSOR_MAJOR>;
     e.hybrid_pass[1]      = &launch_hybrid_pass<K, POLICY_TOPO>;
     e.hybrid_pass[2]      = &launch_hybrid_pass<K, POLICY_SWEEP>;

@@ -118,6 +118,7 @@ context_release(loom_context& x)
     for(void* p: x.hostcol_ptr)
         if(p) (void)hipFree(p);
     if(x.memregs) (void)hipFree(x.memregs);
+    if(x.wgr_next) (void)hipFree(x.wgr_next);
     if(x.jobs) (void)hipFree(x.jobs);
     if(x.bctl) (void)hipFree(x.bctl);
     for(auto& t: x.tickets) (void)hipEventDestroy(t.second);
@@ -316,10 +317,43 @@ run_hybrid_on(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c
     return 0;
 }
 
+/// @brief A run of the workgroup-regrouping kernel: a persistent grid of
+/// resident workgroups, one launch.
+int
+run_wgr(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, const loom_params* p,
+        loom_run_stats* st)
+{
+    const int which = p->reconv == POLICY_SUCCESSOR_MAJOR ? 0 : 1;
+    if(!x.wgr_next) LOOM_HIP(hipMalloc(&x.wgr_next, sizeof(std::uint64_t)));
+    LOOM_HIP(hipMemsetAsync(x.wgr_next, 0, sizeof(std::uint64_t), x.stream));
+    WgrArgs g;
+    g.next  = x.wgr_next;
+    g.phase = p->regroup_phase ? p->regroup_phase : 64;
+    const std::uint64_t need = (c.n + 64ull * kWgrWaves - 1) / (64ull * kWgrWaves);
+    unsigned grid = static_cast<unsigned>(g_dev.cus * e.wgr_occupancy[which]());
+    if(need < grid) grid = static_cast<unsigned>(std::max<std::uint64_t>(1, need));
+    LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
+    LOOM_HIP(e.wgr[which](io, c, g, grid, x.stream));
+    LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
+    LOOM_HIP(hipEventSynchronize(x.ev[1]));
+    float ms = 0;
+    LOOM_HIP(hipEventElapsedTime(&ms, x.ev[0], x.ev[1]));
+    if(st)
+    {
+        st->ms_total      = ms;
+        st->ms_main       = ms;
+        st->main_launches = 1;
+        st->passes        = 1;
+        st->grid_waves    = grid * static_cast<std::uint32_t>(kWgrWaves);
+    }
+    return 0;
+}
+
 int
 run_hybrid(loom_context& x, int kidx, const KernelEntry& e, const IoArgs& io, CtlArgs c,
            const loom_params* p, loom_run_stats* st)
 {
+    if(p && p->regroup == LOOM_REGROUP_WORKGROUP) return run_wgr(x, e, io, c, p, st);
     const int      policy = p ? static_cast<int>(p->reconv) : static_cast<int>(POLICY_SWEEP);
     const unsigned grid   = hybrid_grid(e, policy, c.regroup != 0, p, c.n);
     const std::uint64_t qcap = hybrid_qcap(p, grid, c.n);
@@ -396,6 +430,8 @@ run_on(loom_context& x, int kernel, int mode, const void* const* inputs, void* c
         return fail(LOOM_E_ARG, "a register file in memory runs in the compact form");
     if(memory && scratch)
         return fail(LOOM_E_ARG, "a register file in memory needs an allocation; use loom_run");
+    if(params && params->regroup == LOOM_REGROUP_WORKGROUP && scratch)
+        return fail(LOOM_E_ARG, "workgroup regrouping needs an allocation; use loom_run");
     if(mode == LOOM_MODE_COMPACT)
         r = run_compact(x, e, io, c, params ? static_cast<int>(params->reconv) : 0, memory, stats);
     else if(scratch)
@@ -540,6 +576,8 @@ batch_run_on(loom_context& x, int set, const loom_job* jobs, int njobs, const lo
     if(int r = validate_params(params)) return r;
     if(params && params->register_file != LOOM_REGFILE_VGPR)
         return fail(LOOM_E_ARG, "batched launches keep registers in VGPRs");
+    if(params && params->regroup == LOOM_REGROUP_WORKGROUP)
+        return fail(LOOM_E_ARG, "batched launches do not regroup within workgroups");
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     if(stats) *stats = loom_run_stats{};
@@ -698,6 +736,10 @@ validate_params(const loom_params* params)
         return fail(LOOM_E_ARG, "unknown register file %u", params->register_file);
     if(params && params->register_file == LOOM_REGFILE_MEMORY && params->reconv != POLICY_SWEEP)
         return fail(LOOM_E_ARG, "a register file in memory runs under the sweep policy (reconv 2)");
+    if(params && params->regroup > LOOM_REGROUP_WORKGROUP)
+        return fail(LOOM_E_ARG, "unknown regroup mode %u", params->regroup);
+    if(params && params->regroup == LOOM_REGROUP_WORKGROUP && params->reconv == POLICY_TOPO)
+        return fail(LOOM_E_ARG, "workgroup regrouping runs under reconv 0 or 2");
     return 0;
 }
 
@@ -882,8 +924,9 @@ loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch
     if(int r = check_kernel(kernel)) return r;
     if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
         return fail(LOOM_E_ARG, "unknown mode %d", mode);
-    const int flags = policy & (LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY);
-    policy &= ~(LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY);
+    const int all   = LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY | LOOM_RESOURCES_WORKGROUP;
+    const int flags = policy & all;
+    policy &= ~all;
     if(policy < 0 || policy >= kNumPolicies) return fail(LOOM_E_ARG, "unknown policy %d", policy);
     if(int r = registry()[kernel].kernel_attrs(mode, policy, flags, vgprs, scratch_bytes))
         return fail(r, "hipFuncGetAttributes failed");
@@ -1248,6 +1291,8 @@ loom_batch_graph_create(int set, const loom_job* jobs, int njobs, const loom_par
     if(int r = validate_params(params)) return r;
     if(params && params->register_file != LOOM_REGFILE_VGPR)
         return fail(LOOM_E_ARG, "batched launches keep registers in VGPRs");
+    if(params && params->regroup == LOOM_REGROUP_WORKGROUP)
+        return fail(LOOM_E_ARG, "batched launches do not regroup within workgroups");
     std::vector<JobDesc> jd;
     if(int r = batch_jobs(set, jobs, njobs, jd)) return r;
     // every early return below frees what was made so far (the destructor)

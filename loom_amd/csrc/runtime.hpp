@@ -93,6 +93,9 @@ struct loom_context
     };
     std::vector<Workspace> ws; ///< per kernel index
 
+    /// input cursor of workgroup-regrouping runs
+    std::uint64_t* wgr_next = nullptr;
+
     /// register file of compact runs in device memory, grown on demand
     char*       memregs       = nullptr;
     std::size_t memregs_bytes = 0;

@@ -148,6 +148,8 @@ loom_stream_create(int kernel, uint64_t slots, const loom_params* params, loom_s
     if(params && params->register_file != LOOM_REGFILE_VGPR)
         return fail(LOOM_E_ARG, "a stream keeps its registers in device memory between steps and "
                                 "in VGPRs during them");
+    if(params && params->regroup == LOOM_REGROUP_WORKGROUP)
+        return fail(LOOM_E_ARG, "a stream does not regroup within workgroups");
     auto s    = std::make_unique<loom_stream>();
     s->kernel = kernel;
     s->e      = &kernels()[kernel];

@@ -159,6 +159,13 @@ def resources(name) -> dict:
                                                abi.RECONV_SWEEP | abi.RESOURCES_MEMORY,
                                                C.byref(v), C.byref(sc)), "loom_kernel_resources")
     out["compact-memory/sweep"] = {"vgprs": v.value, "scratch_bytes": sc.value}
+    # the workgroup-regrouping kernel (successor-major, sweep)
+    for pol, pname in ((0, "successor-major"), (2, "sweep")):
+        v, sc = C.c_int(), C.c_int()
+        abi.check(abi.load().loom_kernel_resources(k.index, abi.MODE_HYBRID,
+                                                   pol | abi.RESOURCES_WORKGROUP, C.byref(v),
+                                                   C.byref(sc)), "loom_kernel_resources")
+        out[f"hybrid-workgroup/{pname}"] = {"vgprs": v.value, "scratch_bytes": sc.value}
     return out
 
 

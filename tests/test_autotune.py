@@ -40,8 +40,9 @@ def test_committed_forms_are_valid():
             continue
         f = entry["form"]
         assert f["mode"] in ("hybrid", "compact"), name
-        keys = autotune.SPACE if f["mode"] == "hybrid" else autotune.COMPACT_SPACE
-        assert set(f["params"]) <= set(keys), name
+        keys = (set(autotune.SPACE) | set(autotune.WGR_SPACE) if f["mode"] == "hybrid"
+                else set(autotune.COMPACT_SPACE))
+        assert set(f["params"]) <= keys, name
         # the chosen form is the fastest that keeps the active-lane floor
         assert f["ms"] <= entry["ms"] + 1e-9 or f["active_lane_fraction"] < entry["min_active"]
 
@@ -52,7 +53,8 @@ def test_gpu_tune_small(gpu, tmp_path):
     x = workloads.fib_inputs(1 << 14)
     space = {"live_threshold": [0, 48], "regroup": [False], "reconv": [2], "waves_per_cu": [0]}
     entry = autotune.tune("fib_u64", [x], space=space, verbose=False, min_active=0.0)
-    assert entry["candidates"] == 2 + len(autotune.COMPACT_SPACE["reconv"])
+    assert entry["candidates"] == (2 + len(autotune.COMPACT_SPACE["reconv"])
+                                   + len(autotune.WGR_SPACE["regroup_phase"]))
     f = entry["form"]
     assert f["ms"] == min(f["ms"], entry["ms"])
     m, p = autotune.params_of_form(f["mode"], f["params"])
