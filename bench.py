@@ -161,7 +161,7 @@ def ref_pt2_leg():
     threshold), the hybrid form regrouping the threads of each workgroup by
     block through LDS every WGR_PHASE dispatches, the compact form with its
     register file in memory, and the queue-regrouping form under the
-    reference's successor-major policy at two block-population thresholds
+    reference's successor-major policy at a block-population threshold of 32
     (regrouping passes: a wave stops when its best block has no more lanes
     than the threshold and its threads are queued by label for the next
     pass)."""
@@ -183,8 +183,6 @@ def ref_pt2_leg():
             regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE)),
         "compact_memory_registers": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
                                                             register_file=abi.REGFILE_MEMORY)),
-        "regroup_successor_major_thr8": (abi.MODE_HYBRID, dict(
-            reconv=abi.RECONV_SUCCESSOR_MAJOR, regroup=True, threshold=8, live_threshold=0)),
         "regroup_successor_major_thr32": (abi.MODE_HYBRID, dict(
             reconv=abi.RECONV_SUCCESSOR_MAJOR, regroup=True, threshold=32, live_threshold=0)),
     }

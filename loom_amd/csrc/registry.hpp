@@ -343,8 +343,9 @@ make_entry(const char* name)
     static const bool LOOM_REG_CAT(loom_registered_, LINE) =               \
         ::loom::hipvm::register_kernel(::loom::hipvm::make_entry<__VA_ARGS__>(NAME))
 
-/// @brief Compile the hybrid pass kernels of kernel type `...` for at least
-/// @p W waves per SIMD (before its LOOM_HIP_REGISTER, at namespace scope).
+/// @brief Compile the hybrid pass kernel (without regroup paths) of kernel
+/// type `...` for at least @p W waves per SIMD (before its
+/// LOOM_HIP_REGISTER, at namespace scope).
 #define LOOM_HIP_WAVES_HINT(W, ...)                                                   \
     template <>                                                                        \
     inline constexpr int ::loom::hipvm::waves_hint<__VA_ARGS__> = (W)

@@ -10,12 +10,14 @@ pytestmark = pytest.mark.gpu
 
 
 # kernels registered with an occupancy hint (LOOM_HIP_WAVES_HINT) and the
-# scratch bytes per lane that holding their pass kernels to it may cost: the
-# path tracer is held to 128 VGPRs, two values spilled; the reference's
-# pt2_const (registered when its library is loaded) spills nothing without
-# the regroup paths and 324 bytes with them (239 VGPRs left to itself)
-HINTED_SCRATCH = {"smallpt9_f32": 16, "ref_pt2_const_b5_t0.0005_d20": 512,
-                  "ref_pt2_const_b5_t0.0005_d20_1024": 512}
+# scratch bytes per lane that 128 VGPRs may cost them: the path tracer's pass
+# kernel is held to 128, two values spilled; the reference's pt2_const
+# (registered when its library is loaded) spills nothing under the sweep
+# policy and up to 160 bytes under the other two, and its 1024-thread
+# workgroup-regrouping kernel (128 VGPRs at most) 188 bytes under
+# successor-major
+HINTED_SCRATCH = {"smallpt9_f32": 16, "ref_pt2_const_b5_t0.0005_d20": 192,
+                  "ref_pt2_const_b5_t0.0005_d20_1024": 192}
 
 
 def test_no_kernel_uses_scratch(gpu):
@@ -25,7 +27,8 @@ def test_no_kernel_uses_scratch(gpu):
         for form, r in res.items():
             assert 0 < r["vgprs"] <= 512, (k.name, form, r)
             assert r["scratch_bytes"] <= HINTED_SCRATCH.get(k.name, 0), (k.name, form, r)
-            if k.name in HINTED_SCRATCH and form.startswith("hybrid"):
+            if k.name in HINTED_SCRATCH and form in ("hybrid/sweep", "hybrid/topo",
+                                                     "hybrid/successor-major"):
                 assert r["vgprs"] <= 128, (k.name, form, r)
 
 

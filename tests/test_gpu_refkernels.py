@@ -24,6 +24,9 @@ FORMS = [
     ("hybrid_sweep", abi.MODE_HYBRID, {}),
     ("hybrid_sm_regroup", abi.MODE_HYBRID, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR,
                                                 regroup=True, live_threshold=32)),
+    ("hybrid_workgroup_regroup", abi.MODE_HYBRID, dict(regroup=abi.REGROUP_WORKGROUP,
+                                                       reconv=abi.RECONV_SWEEP,
+                                                       regroup_phase=128)),
 ]
 # published single-core Zen 4 frame times, s: predicated (compact), hybrid
 # (reference site-docs/benchmarks/results/a0d934323c/5c42ee70d3/*.csv)
@@ -73,7 +76,9 @@ def test_reference_kernel_gpu_sized_frame(gpu):
     forms = [("compact_sweep", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
              ("compact_memory", abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
                                                        register_file=abi.REGFILE_MEMORY)),
-             ("hybrid_sweep", abi.MODE_HYBRID, {})]
+             ("hybrid_sweep", abi.MODE_HYBRID, {}),
+             ("hybrid_workgroup_regroup", abi.MODE_HYBRID,
+              dict(regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=128))]
     for fname, mode, params in forms:
         got = loom_amd.run(k, mode, [pick], params=loom_amd.make_params(mode=mode, **params))
         assert got.errors == 0

@@ -27,6 +27,9 @@ FORMS = [
     ("hybrid_stream_1wpc", abi.MODE_HYBRID, dict(waves_per_cu=1, live_threshold=0)),
     ("hybrid_sm_regroup", abi.MODE_HYBRID, dict(reconv=abi.RECONV_SUCCESSOR_MAJOR,
                                                 regroup=True, live_threshold=32)),
+    ("hybrid_workgroup_regroup", abi.MODE_HYBRID, dict(regroup=abi.REGROUP_WORKGROUP,
+                                                       reconv=abi.RECONV_SWEEP,
+                                                       regroup_phase=16)),
 ]
 
 
