@@ -159,7 +159,8 @@ def ref_pt2_leg():
     this backend (oracle/build_ref.py): frame time, rays/s and active lanes
     of the compact sweep form, the hybrid form (refill, tuned live
     threshold), the hybrid form regrouping the threads of each workgroup by
-    block through LDS every WGR_PHASE dispatches, the compact form with its
+    block through LDS every WGR_PHASE dispatches (16-wave workgroups, one per
+    CU, and 8-wave ones, two per CU), the compact form with its
     register file in memory, and the queue-regrouping form under the
     reference's successor-major policy at a block-population threshold of 32
     (regrouping passes: a wave stops when its best block has no more lanes
@@ -181,6 +182,9 @@ def ref_pt2_leg():
         "hybrid_live56": (abi.MODE_HYBRID, dict(live_threshold=56)),
         "hybrid_workgroup_regroup": (abi.MODE_HYBRID, dict(
             regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE)),
+        "hybrid_workgroup8_regroup": (abi.MODE_HYBRID, dict(
+            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE,
+            regroup_waves=8)),
         "compact_memory_registers": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
                                                             register_file=abi.REGFILE_MEMORY)),
         "regroup_successor_major_thr32": (abi.MODE_HYBRID, dict(

@@ -124,6 +124,9 @@ typedef struct loom_params
                                 sweep policy only) */
     uint32_t regroup_phase;  /* LOOM_REGROUP_WORKGROUP: dispatches of a run
                                 phase between two sorts (0: 64) */
+    uint32_t regroup_waves;  /* LOOM_REGROUP_WORKGROUP: waves per workgroup,
+                                16 (one workgroup per CU) or 8 (two per CU;
+                                one sorts while the other runs); 0: 16 */
 } loom_params;
 #define LOOM_REGFILE_VGPR 0
 #define LOOM_REGFILE_MEMORY 1
@@ -171,6 +174,7 @@ int loom_kernel_dump(int kernel, char* buf, size_t len, size_t* needed);
 #define LOOM_RESOURCES_REGROUP 0x100
 #define LOOM_RESOURCES_MEMORY 0x200
 #define LOOM_RESOURCES_WORKGROUP 0x400 /* the workgroup-regrouping kernel (reconv 0 or 2) */
+#define LOOM_RESOURCES_WORKGROUP_8 0x800 /* the same with 8-wave workgroups */
 int loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch_bytes);
 /* Register r (0 <= r < n_registers) of a kernel: its name (NUL-terminated,
  * at most len bytes), first type (loom_type), width in bytes, number of

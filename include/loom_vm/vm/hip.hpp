@@ -43,7 +43,9 @@ struct params
     std::uint32_t threshold      = 0;
     std::uint32_t live_threshold = 48;
     bool          refill         = true;
-    bool          regroup        = false;
+    /// hybrid: LOOM_REGROUP_NONE (false), LOOM_REGROUP_QUEUES (true) or
+    /// LOOM_REGROUP_WORKGROUP
+    std::uint32_t regroup        = LOOM_REGROUP_NONE;
     int           reconv         = -1; ///< -1: per mode default
     std::uint32_t waves_per_cu   = 0;
     std::uint64_t queue_capacity = 0;
@@ -51,6 +53,10 @@ struct params
     /// compact_function: keep the registers in device memory instead of
     /// VGPRs (LOOM_REGFILE_MEMORY; runs under the sweep policy)
     std::uint32_t register_file  = LOOM_REGFILE_VGPR;
+    /// LOOM_REGROUP_WORKGROUP: dispatches between two sorts (0: 64) and
+    /// waves per workgroup (16 or 8; 0: 16)
+    std::uint32_t regroup_phase  = 0;
+    std::uint32_t regroup_waves  = 0;
 };
 
 namespace detail
@@ -94,6 +100,8 @@ to_c(const params& p, int mode)
     c.waves_per_cu   = p.waves_per_cu;
     c.queue_capacity = p.queue_capacity;
     c.max_steps      = p.max_steps;
+    c.regroup_phase  = p.regroup_phase;
+    c.regroup_waves  = p.regroup_waves;
     return c;
 }
 

@@ -168,14 +168,15 @@ class RunResult:
 def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, waves_per_cu=0,
                 queue_capacity=0, max_steps=0, stats=True, reconv=None,
                 mode=abi.MODE_COMPACT, register_file=abi.REGFILE_VGPR,
-                regroup_phase=0) -> abi.Params:
+                regroup_phase=0, regroup_waves=0) -> abi.Params:
     """Run parameters; `reconv` defaults to successor-major for the compact
     form (the reference policy) and to the sweep policy for hybrid,
     `live_threshold` to 48 for hybrid (0 for compact, which does not
     refill); `register_file` = REGFILE_MEMORY keeps a compact run's
     registers in device memory (sweep policy); `regroup` = REGROUP_WORKGROUP
     sorts a workgroup's threads by block between run phases of
-    `regroup_phase` dispatches."""
+    `regroup_phase` dispatches, in workgroups of `regroup_waves` waves (16,
+    one per CU, or 8, two per CU; 0 = 16)."""
     p = abi.Params()
     if reconv is None:
         reconv = abi.RECONV_SUCCESSOR_MAJOR if mode == abi.MODE_COMPACT else abi.RECONV_SWEEP
@@ -192,6 +193,7 @@ def make_params(threshold=0, live_threshold=None, refill=True, regroup=False, wa
     p.collect_stats = 1 if stats else 0
     p.register_file = int(register_file)
     p.regroup_phase = int(regroup_phase)
+    p.regroup_waves = int(regroup_waves)
     return p
 
 

@@ -21,6 +21,7 @@ MODE_HYBRID = 1
 RESOURCES_REGROUP = 0x100
 RESOURCES_MEMORY = 0x200
 RESOURCES_WORKGROUP = 0x400
+RESOURCES_WORKGROUP_8 = 0x800
 REGROUP_NONE = 0
 REGROUP_QUEUES = 1
 REGROUP_WORKGROUP = 2
@@ -84,6 +85,7 @@ class Params(C.Structure):
         ("max_steps", C.c_uint64),
         ("register_file", C.c_uint32),
         ("regroup_phase", C.c_uint32),
+        ("regroup_waves", C.c_uint32),
     ]
 
 

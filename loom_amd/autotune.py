@@ -45,8 +45,8 @@ SPACE = {
 COMPACT_SPACE = {"reconv": [0, 1, 2]}
 
 # the hybrid form regrouping within workgroups (sweep policy): the phase
-# length between two sorts
-WGR_SPACE = {"regroup_phase": [64, 256, 1024]}
+# length between two sorts and the workgroup size in waves
+WGR_SPACE = {"regroup_phase": [64, 256, 1024], "regroup_waves": [16, 8]}
 
 
 def best_params(kernel_name: str) -> dict:

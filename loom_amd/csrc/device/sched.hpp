@@ -35,17 +35,20 @@
 namespace loom::hipvm
 {
 /// @brief Waves per SIMD that the hybrid pass kernel of @p K without the
-/// regroup paths (and its workgroup-regrouping kernel) is compiled to allow
-/// at least (amdgpu_waves_per_eu): 1 leaves the register budget to the
-/// compiler.  A kernel whose register file sits just above a budget step
-/// (128 VGPRs for 4 waves) can run faster held under it, a few values
-/// spilled, than at one wave less; the registration of such a kernel sets
-/// its hint (LOOM_HIP_WAVES_HINT).  The compact kernels and the pass kernel
-/// with the regroup paths are left to the compiler: they can need far more
-/// registers (the fill and spill switches keep every label's registers
-/// live), which a hint would spill.
+/// regroup paths is compiled to allow at least (amdgpu_waves_per_eu): 1
+/// leaves the register budget to the compiler.  A kernel whose register file
+/// sits just above a budget step (128 VGPRs for 4 waves, 96 for 5) can run
+/// faster held under it, a few values spilled, than at one wave less; the
+/// registration of such a kernel sets its hint (LOOM_HIP_WAVES_HINT).  The
+/// compact kernels are left to the compiler: under the successor-major
+/// policy they can need far more registers, which a hint would spill.
 template <typename K>
 inline constexpr int waves_hint = 1;
+/// @brief The same for the pass kernel with the regroup paths, whose fill
+/// and spill switches keep more registers live, so that a hint that pays
+/// without them can spill too much with them (LOOM_HIP_REGROUP_WAVES_HINT).
+template <typename K>
+inline constexpr int regroup_waves_hint = 1;
 
 inline constexpr int         kMaxIO    = 32;
 inline constexpr std::size_t kNumStats = 13;
@@ -1073,7 +1076,7 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 /// pt2_const: 239 -> 138, two -> three waves per SIMD), so runs that do not
 /// regroup launch the RG = false kernel, in which no queue is ever read.
 template <typename K, int P, bool RG>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RG ? 1 : waves_hint<K>))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RG ? regroup_waves_hint<K> : waves_hint<K>))) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
     HybridCtl* ctl            = h.ctl;

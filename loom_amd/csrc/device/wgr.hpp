@@ -8,8 +8,9 @@
 /// Reference: the hybrid compiler's regroup (src/vm/simd.hpp:4092-4252), a
 /// stable counting sort of the whole pool by program counter with halted
 /// rows last (simd.hpp:2667-2745), every register column gathered and
-/// scattered.  Here the pool is one workgroup of WAVES waves (1024 threads
-/// with 16 waves, every lane of a CU when a lane needs at most 128 VGPRs):
+/// scattered.  Here the pool is one workgroup of W waves (1024 threads
+/// with 16 waves, every lane of a CU when a lane needs at most 128 VGPRs;
+/// 512 with 8, two workgroups per CU):
 ///
 ///   sort      every lane's key is its block (HALT last); an LDS histogram
 ///             (one atomic per distinct key of a wave) and its prefix give
@@ -39,7 +40,9 @@ namespace loom::hipvm
 inline constexpr std::uint32_t kNoTid = 0xFFFFFFFFu;
 /// @brief State words exchanged per thread per LDS round.
 inline constexpr std::size_t kWgrRoundWords = 15;
-/// @brief Waves of a regrouping workgroup.
+/// @brief Default waves of a regrouping workgroup: 16 (one workgroup holds
+/// every lane of a CU at four waves per SIMD); the other size is 8 (two
+/// workgroups per CU, so one runs while the other sorts).
 inline constexpr int kWgrWaves = 16;
 
 struct WgrArgs
@@ -190,14 +193,18 @@ wgr_round(Regs<K>& r, std::uint32_t old_key, std::uint32_t new_key, std::uint32_
     __syncthreads();
 }
 
-/// @brief The regrouping kernel (see the file comment).  A persistent grid:
-/// every workgroup loops until the input is exhausted and its own threads
-/// have finished; workgroups never wait for one another.
-template <typename K, int P>
-__global__ __launch_bounds__(64 * kWgrWaves) __attribute__((amdgpu_waves_per_eu(waves_hint<K>))) void
+/// @brief The regrouping kernel (see the file comment) with workgroups of
+/// @p W waves.  A persistent grid: every workgroup loops until the input is
+/// exhausted and its own threads have finished; workgroups never wait for
+/// one another.  Both sizes are compiled for four waves per SIMD (128
+/// VGPRs), which the 16-wave workgroup needs to be resident at all and
+/// which lets two 8-wave workgroups share a CU.
+template <typename K, int P, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void
 wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
 {
-    constexpr std::uint32_t T    = 64u * kWgrWaves;
+    static_assert(W == 16 || W == 8, "a regrouping workgroup has 16 or 8 waves");
+    constexpr std::uint32_t T    = 64u * W;
     constexpr std::uint32_t NB   = ir::table_of<K>.nb;
     constexpr std::uint32_t KEYS = NB + 1; // blocks, then "no live thread"
     constexpr std::size_t   NR   = (WgrWords<K>::n + kWgrRoundWords - 1) / kWgrRoundWords;

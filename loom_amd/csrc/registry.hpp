@@ -34,11 +34,11 @@ struct KernelEntry
     /// the compact kernel with its register file in device memory (sweep
     /// policy; MemArgs.rows >= the run's threads)
     hipError_t (*compact_mem)(const IoArgs&, const CtlArgs&, const MemArgs&, hipStream_t) = nullptr;
-    /// the workgroup-regrouping kernel (successor-major, sweep) and its
-    /// resident workgroups per CU
-    hipError_t (*wgr[2])(const IoArgs&, const CtlArgs&, const WgrArgs&, unsigned grid,
-                         hipStream_t) = {};
-    int (*wgr_occupancy[2])() = {};
+    /// the workgroup-regrouping kernel (successor-major, sweep; workgroups
+    /// of `waves` = 16 or 8 waves) and its resident workgroups per CU
+    hipError_t (*wgr[2])(const IoArgs&, const CtlArgs&, const WgrArgs&, int waves,
+                         unsigned grid, hipStream_t) = {};
+    int (*wgr_occupancy[2])(int waves) = {};
     hipError_t (*hybrid_pass[kNumPolicies])(const IoArgs&, const CtlArgs&, const HybridArgs&,
                                  unsigned grid, hipStream_t)           = {};
     /// resident 64-thread workgroups per CU of the hybrid pass kernel
@@ -140,21 +140,28 @@ launch_compact_mem(const IoArgs& io, const CtlArgs& c, const MemArgs& m, hipStre
 
 template <typename K, int P>
 hipError_t
-launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, unsigned grid, hipStream_t s)
+launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, int waves, unsigned grid,
+           hipStream_t s)
 {
     if(c.n == 0) return hipSuccess;
-    hipLaunchKernelGGL((wgr_kernel<K, P>), dim3(grid), dim3(64 * kWgrWaves), 0, s, io, c, g);
+    if(waves == 8)
+        hipLaunchKernelGGL((wgr_kernel<K, P, 8>), dim3(grid), dim3(64 * 8), 0, s, io, c, g);
+    else
+        hipLaunchKernelGGL((wgr_kernel<K, P, 16>), dim3(grid), dim3(64 * 16), 0, s, io, c, g);
     return hipGetLastError();
 }
 
 template <typename K, int P>
 int
-wgr_occupancy()
+wgr_occupancy(int waves)
 {
-    int nblk = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, wgr_kernel<K, P>, 64 * kWgrWaves, 0) !=
-       hipSuccess)
-        return 1;
+    int              nblk = 0;
+    const hipError_t e =
+        waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, wgr_kernel<K, P, 8>,
+                                                                  64 * 8, 0) :
+                     hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, wgr_kernel<K, P, 16>,
+                                                                  64 * 16, 0);
+    if(e != hipSuccess) return 1;
     return nblk > 0 ? nblk : 1;
 }
 
@@ -197,13 +204,15 @@ kernel_attrs(int mode, int policy, int flags, int* vgprs, int* scratch)
     const bool  regroup = (flags & LOOM_RESOURCES_REGROUP) != 0;
     const bool  memory  = (flags & LOOM_RESOURCES_MEMORY) != 0;
     const bool  wgr     = (flags & LOOM_RESOURCES_WORKGROUP) != 0;
+    const bool  wgr8    = (flags & LOOM_RESOURCES_WORKGROUP_8) != 0;
     const void* f = nullptr;
     auto        pick = [&]<int P>()
     {
-        if(mode == LOOM_MODE_HYBRID && wgr)
-            f = P == POLICY_SUCCESSOR_MAJOR ?
-                    reinterpret_cast<const void*>(wgr_kernel<K, POLICY_SUCCESSOR_MAJOR>) :
-                    reinterpret_cast<const void*>(wgr_kernel<K, POLICY_SWEEP>);
+        constexpr int WP = P == POLICY_SUCCESSOR_MAJOR ? POLICY_SUCCESSOR_MAJOR : POLICY_SWEEP;
+        if(mode == LOOM_MODE_HYBRID && wgr8)
+            f = reinterpret_cast<const void*>(wgr_kernel<K, WP, 8>);
+        else if(mode == LOOM_MODE_HYBRID && wgr)
+            f = reinterpret_cast<const void*>(wgr_kernel<K, WP, 16>);
         else if(mode == LOOM_MODE_COMPACT && memory)
             f = reinterpret_cast<const void*>(compact_mem_kernel<K, POLICY_SWEEP>);
         else if(mode == LOOM_MODE_COMPACT)
@@ -343,12 +352,17 @@ make_entry(const char* name)
     static const bool LOOM_REG_CAT(loom_registered_, LINE) =               \
         ::loom::hipvm::register_kernel(::loom::hipvm::make_entry<__VA_ARGS__>(NAME))
 
-/// @brief Compile the hybrid pass kernel (without regroup paths) of kernel
-/// type `...` for at least @p W waves per SIMD (before its
-/// LOOM_HIP_REGISTER, at namespace scope).
+/// @brief Compile the hybrid pass kernel without the regroup paths of kernel
+/// type `...` for at least @p W waves per SIMD (before its LOOM_HIP_REGISTER,
+/// at namespace scope).
 #define LOOM_HIP_WAVES_HINT(W, ...)                                                   \
     template <>                                                                        \
     inline constexpr int ::loom::hipvm::waves_hint<__VA_ARGS__> = (W)
+
+/// @brief The same for the pass kernel with the regroup paths.
+#define LOOM_HIP_REGROUP_WAVES_HINT(W, ...)                                           \
+    template <>                                                                        \
+    inline constexpr int ::loom::hipvm::regroup_waves_hint<__VA_ARGS__> = (W)
 
 /// @brief Lower kernel type `...` and register it under @p NAME.
 #define LOOM_HIP_REGISTER(NAME, ...) LOOM_HIP_REGISTER_I(__COUNTER__, NAME, __VA_ARGS__)

@@ -324,16 +324,18 @@ run_wgr(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, cons
         loom_run_stats* st)
 {
     const int which = p->reconv == POLICY_SUCCESSOR_MAJOR ? 0 : 1;
+    const int waves = p->regroup_waves == 8 ? 8 : kWgrWaves;
     if(!x.wgr_next) LOOM_HIP(hipMalloc(&x.wgr_next, sizeof(std::uint64_t)));
     LOOM_HIP(hipMemsetAsync(x.wgr_next, 0, sizeof(std::uint64_t), x.stream));
     WgrArgs g;
     g.next  = x.wgr_next;
     g.phase = p->regroup_phase ? p->regroup_phase : 64;
-    const std::uint64_t need = (c.n + 64ull * kWgrWaves - 1) / (64ull * kWgrWaves);
-    unsigned grid = static_cast<unsigned>(g_dev.cus * e.wgr_occupancy[which]());
+    const std::uint64_t lanes = 64ull * static_cast<std::uint64_t>(waves);
+    const std::uint64_t need  = (c.n + lanes - 1) / lanes;
+    unsigned grid = static_cast<unsigned>(g_dev.cus * e.wgr_occupancy[which](waves));
     if(need < grid) grid = static_cast<unsigned>(std::max<std::uint64_t>(1, need));
     LOOM_HIP(hipEventRecord(x.ev[0], x.stream));
-    LOOM_HIP(e.wgr[which](io, c, g, grid, x.stream));
+    LOOM_HIP(e.wgr[which](io, c, g, waves, grid, x.stream));
     LOOM_HIP(hipEventRecord(x.ev[1], x.stream));
     LOOM_HIP(hipEventSynchronize(x.ev[1]));
     float ms = 0;
@@ -344,7 +346,7 @@ run_wgr(loom_context& x, const KernelEntry& e, const IoArgs& io, CtlArgs c, cons
         st->ms_main       = ms;
         st->main_launches = 1;
         st->passes        = 1;
-        st->grid_waves    = grid * static_cast<std::uint32_t>(kWgrWaves);
+        st->grid_waves    = grid * static_cast<std::uint32_t>(waves);
     }
     return 0;
 }
@@ -740,6 +742,10 @@ validate_params(const loom_params* params)
         return fail(LOOM_E_ARG, "unknown regroup mode %u", params->regroup);
     if(params && params->regroup == LOOM_REGROUP_WORKGROUP && params->reconv == POLICY_TOPO)
         return fail(LOOM_E_ARG, "workgroup regrouping runs under reconv 0 or 2");
+    if(params && params->regroup_waves != 0 && params->regroup_waves != 8 &&
+       params->regroup_waves != 16)
+        return fail(LOOM_E_ARG, "a regrouping workgroup has 16 or 8 waves, not %u",
+                    params->regroup_waves);
     return 0;
 }
 
@@ -924,7 +930,8 @@ loom_kernel_resources(int kernel, int mode, int policy, int* vgprs, int* scratch
     if(int r = check_kernel(kernel)) return r;
     if(mode != LOOM_MODE_COMPACT && mode != LOOM_MODE_HYBRID)
         return fail(LOOM_E_ARG, "unknown mode %d", mode);
-    const int all   = LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY | LOOM_RESOURCES_WORKGROUP;
+    const int all   = LOOM_RESOURCES_REGROUP | LOOM_RESOURCES_MEMORY | LOOM_RESOURCES_WORKGROUP |
+                    LOOM_RESOURCES_WORKGROUP_8;
     const int flags = policy & all;
     policy &= ~all;
     if(policy < 0 || policy >= kNumPolicies) return fail(LOOM_E_ARG, "unknown policy %d", policy);

@@ -159,13 +159,16 @@ def resources(name) -> dict:
                                                abi.RECONV_SWEEP | abi.RESOURCES_MEMORY,
                                                C.byref(v), C.byref(sc)), "loom_kernel_resources")
     out["compact-memory/sweep"] = {"vgprs": v.value, "scratch_bytes": sc.value}
-    # the workgroup-regrouping kernel (successor-major, sweep)
-    for pol, pname in ((0, "successor-major"), (2, "sweep")):
-        v, sc = C.c_int(), C.c_int()
-        abi.check(abi.load().loom_kernel_resources(k.index, abi.MODE_HYBRID,
-                                                   pol | abi.RESOURCES_WORKGROUP, C.byref(v),
-                                                   C.byref(sc)), "loom_kernel_resources")
-        out[f"hybrid-workgroup/{pname}"] = {"vgprs": v.value, "scratch_bytes": sc.value}
+    # the workgroup-regrouping kernel (successor-major, sweep; 16- and
+    # 8-wave workgroups)
+    for flag, wname in ((abi.RESOURCES_WORKGROUP, "hybrid-workgroup"),
+                        (abi.RESOURCES_WORKGROUP_8, "hybrid-workgroup8")):
+        for pol, pname in ((0, "successor-major"), (2, "sweep")):
+            v, sc = C.c_int(), C.c_int()
+            abi.check(abi.load().loom_kernel_resources(k.index, abi.MODE_HYBRID, pol | flag,
+                                                       C.byref(v), C.byref(sc)),
+                      "loom_kernel_resources")
+            out[f"{wname}/{pname}"] = {"vgprs": v.value, "scratch_bytes": sc.value}
     return out
 
 

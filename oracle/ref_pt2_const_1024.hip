@@ -7,7 +7,10 @@
 #include "ref_kernels.hpp"
 
 // The pass kernel needs 138 VGPRs left to itself (three waves per SIMD);
-// held to 128 it spills nothing and runs four waves per SIMD.
+// held to 128 it spills nothing and runs four waves per SIMD.  With the
+// regroup paths (239 VGPRs) it is left to itself: held to 128 it spills 324
+// bytes per lane and the queue-regrouping run of the 1024x1024 frame takes
+// 36 s instead of 24.
 LOOM_HIP_WAVES_HINT(4, loom::vm::pt2_const<float, loom_ref::c_b5_t0005_d20_1024>);
 
 #define LOOM_REGISTER_ONE(NAME, ...) LOOM_HIP_REGISTER(NAME, __VA_ARGS__);

@@ -54,7 +54,8 @@ def test_gpu_tune_small(gpu, tmp_path):
     space = {"live_threshold": [0, 48], "regroup": [False], "reconv": [2], "waves_per_cu": [0]}
     entry = autotune.tune("fib_u64", [x], space=space, verbose=False, min_active=0.0)
     assert entry["candidates"] == (2 + len(autotune.COMPACT_SPACE["reconv"])
-                                   + len(autotune.WGR_SPACE["regroup_phase"]))
+                                   + len(autotune.WGR_SPACE["regroup_phase"])
+                                   * len(autotune.WGR_SPACE["regroup_waves"]))
     f = entry["form"]
     assert f["ms"] == min(f["ms"], entry["ms"])
     m, p = autotune.params_of_form(f["mode"], f["params"])

@@ -1,9 +1,10 @@
 """Regrouping inside a workgroup (loom_params.regroup = LOOM_REGROUP_WORKGROUP):
-the 1024 threads of a workgroup are sorted by block through LDS between run
-phases, their live registers moving with them.  Every output equals the
-oracle's bit for bit, and the useful work the device counts (active
-operations and flops, thread-block steps) equals the compact form's: the
-same blocks run on the same threads, only grouped differently."""
+the 1024 (or, with 8-wave workgroups, 512) threads of a workgroup are sorted
+by block through LDS between run phases, their live registers moving with
+them.  Every output equals the oracle's bit for bit, and the useful work the
+device counts (active operations and flops, thread-block steps) equals the
+compact form's: the same blocks run on the same threads, only grouped
+differently."""
 import numpy as np
 import pytest
 
@@ -31,7 +32,12 @@ FORMS = [("successor-major thr16 phase64", dict(reconv=abi.RECONV_SUCCESSOR_MAJO
                                                 regroup_phase=64)),
          ("successor-major thr0 phase8", dict(reconv=abi.RECONV_SUCCESSOR_MAJOR, threshold=0,
                                               regroup_phase=8)),
-         ("sweep phase32", dict(reconv=abi.RECONV_SWEEP, regroup_phase=32))]
+         ("sweep phase32", dict(reconv=abi.RECONV_SWEEP, regroup_phase=32)),
+         ("8 waves successor-major thr8 phase16", dict(reconv=abi.RECONV_SUCCESSOR_MAJOR,
+                                                       threshold=8, regroup_phase=16,
+                                                       regroup_waves=8)),
+         ("8 waves sweep phase64", dict(reconv=abi.RECONV_SWEEP, regroup_phase=64,
+                                        regroup_waves=8))]
 WORK = (abi.C_ACTIVE_OPS, abi.C_THREAD_STEPS, abi.C_ACTIVE_FLOPS)
 
 
@@ -57,13 +63,15 @@ def test_workgroup_regroup_matches_oracle(gpu, name):
 def test_workgroup_regroup_small_and_empty(gpu):
     """Fewer threads than one workgroup's lanes."""
     k = loom_amd.kernel("collatz_u64")
-    for n in (1, 63, 1000):
-        x = [workloads.collatz_inputs(n, seed=86 + n)]
-        want, _ = O.run("collatz_u64", x)
-        p = loom_amd.make_params(mode=abi.MODE_HYBRID, regroup=abi.REGROUP_WORKGROUP,
-                                 reconv=abi.RECONV_SWEEP)
-        r = loom_amd.run(k, abi.MODE_HYBRID, x, params=p)
-        assert np.array_equal(r.outputs[0], want[0]), n
+    for waves in (16, 8):
+        for n in (1, 63, 511, 1000):
+            x = [workloads.collatz_inputs(n, seed=86 + n)]
+            want, _ = O.run("collatz_u64", x)
+            p = loom_amd.make_params(mode=abi.MODE_HYBRID, regroup=abi.REGROUP_WORKGROUP,
+                                     reconv=abi.RECONV_SWEEP, regroup_waves=waves)
+            r = loom_amd.run(k, abi.MODE_HYBRID, x, params=p)
+            assert np.array_equal(r.outputs[0], want[0]), (waves, n)
+            assert r.grid_waves % waves == 0, (waves, n)
 
 
 def test_workgroup_regroup_rejected_with_topo(gpu):
@@ -72,3 +80,12 @@ def test_workgroup_regroup_rejected_with_topo(gpu):
                              reconv=abi.RECONV_TOPO)
     with pytest.raises(abi.LoomError):
         loom_amd.run(k, abi.MODE_HYBRID, [workloads.fib_inputs(100)], params=p)
+
+
+def test_workgroup_regroup_rejects_other_sizes(gpu):
+    k = loom_amd.kernel("fib_u64")
+    for waves in (4, 12, 32):
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, regroup=abi.REGROUP_WORKGROUP,
+                                 reconv=abi.RECONV_SWEEP, regroup_waves=waves)
+        with pytest.raises(abi.LoomError):
+            loom_amd.run(k, abi.MODE_HYBRID, [workloads.fib_inputs(100)], params=p)
