@@ -206,25 +206,31 @@ def ref_pt2_leg():
     return out
 
 
-def regroup_leg(k, ins, outs, h_img):
-    """The queue-regrouping form on the same frame: no in-wave refill, every
-    pass spills the survivors of each wave to the queue of their next label
-    (one atomic per wave and label, rows ranked with mbcnt) and the next pass
-    runs 64-row batches of one label.  Reports the regroup traffic (spilled
-    and filled row bytes) over the pass kernels' time."""
+def regroup_leg(k, ins, outs, h_img, hyb):
+    """The queue-regrouping form on the same frame: every pass spills the
+    survivors of each wave to the queue of their next label (one atomic per
+    wave and label, rows ranked with mbcnt) and the next pass runs 64-row
+    batches of one label.  `live32` and `live48` regroup without in-wave
+    refill (a wave stops at that many live lanes and spills them);
+    `with_refill` refills free lanes while input lasts, with the tuned live
+    threshold and grid, and regroups the survivors once the input is
+    exhausted.  Reports the regroup traffic (spilled and filled row bytes)
+    over the pass kernels' time."""
     import loom_amd
     from loom_amd import abi
     out = {}
-    for live in (32, 48):
-        p = loom_amd.make_params(mode=abi.MODE_HYBRID, live_threshold=live, refill=False,
-                                 regroup=True)
+    forms = {f"live{live}": dict(live_threshold=live, refill=False) for live in (32, 48)}
+    forms["with_refill"] = dict(live_threshold=hyb["live_threshold"],
+                                waves_per_cu=hyb["waves_per_cu"], refill=True)
+    for name, kw in forms.items():
+        p = loom_amd.make_params(mode=abi.MODE_HYBRID, regroup=True, **kw)
         loom_amd.run(k, abi.MODE_HYBRID, ins, outs, params=p)
         r = loom_amd.run(k, abi.MODE_HYBRID, ins, outs, params=p)
         same = all(np.array_equal(o.numpy().view(np.uint32), x.view(np.uint32))
                    for o, x in zip(outs, h_img))
         moved = r.counters[abi.C_SPILL_BYTES] + r.counters[abi.C_FILL_BYTES]
         gbs = moved / (r.ms_main / 1e3) / 1e9
-        out[f"live{live}"] = {
+        out[name] = {
             "ms_total": r.ms_total, "ms_passes": r.ms_main, "passes": r.passes,
             "samples_per_s": W * H * SPP / (r.ms_total / 1e3),
             "active_lane_fraction": r.active_lane_fraction,
@@ -455,7 +461,12 @@ def main():
                 "attainable ceiling"}
 
     if not a.no_regroup:
-        result["regroup"] = regroup_leg(k, ins, outs, h_img)
+        rg = regroup_leg(k, ins, outs, h_img, hyb)
+        for key in ("compact", "compact_sweep"):
+            if key in result:
+                rg["with_refill"][f"over_{key}"] = (result[key]["ms_per_step"]
+                                                    / rg["with_refill"]["ms_total"])
+        result["regroup"] = rg
     if not a.no_int:
         result["integer_configs"] = integer_configs(a.steps)
     if not a.no_ref_pt2:

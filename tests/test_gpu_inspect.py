@@ -11,12 +11,12 @@ pytestmark = pytest.mark.gpu
 
 # kernels registered with an occupancy hint (LOOM_HIP_WAVES_HINT) and the
 # scratch bytes per lane that the hint may cost them: the path tracer's pass
-# kernel is held to 96 VGPRs (five waves per SIMD), 32-40 bytes spilled, and
-# to 128 with the regroup paths; the reference's pt2_const (registered when
-# its library is loaded) is held to 128 without the regroup paths: nothing
-# spilled under the sweep policy, up to 160 bytes under the other two; its
-# workgroup-regrouping kernels (128 VGPRs at most) spill 188 bytes under
-# successor-major
+# kernels are held to 96 VGPRs (five waves per SIMD), 32-40 bytes spilled,
+# with and without the regroup paths; the reference's pt2_const (registered
+# when its library is loaded) is held to 128 without the regroup paths:
+# nothing spilled under the sweep policy, up to 160 bytes under the other
+# two; its workgroup-regrouping kernels (128 VGPRs at most) spill 188 bytes
+# under successor-major
 HINTED_SCRATCH = {"smallpt9_f32": 48, "ref_pt2_const_b5_t0.0005_d20": 192,
                   "ref_pt2_const_b5_t0.0005_d20_1024": 192}
 
