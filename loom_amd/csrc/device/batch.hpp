@@ -147,6 +147,7 @@ template <int P, typename... Ks>
 __global__ __launch_bounds__(64) void
 batch_kernel(CtlArgs c, BatchArgs a)
 {
+    (sct_table_init<Ks>(), ...);
     WaveCtx w(c);
     std::uint64_t admitted = 0;
     const std::uint32_t nj = a.njobs;

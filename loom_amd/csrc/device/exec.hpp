@@ -215,6 +215,89 @@ exact_reciprocal_bits()
     }
 }
 
+/// @brief Constant tables of `sct` (select a constant by index) in LDS.
+///
+/// `sct(i, c0, ..., cN-1)` is c_i for 0 <= i < N and 0 otherwise
+/// (operators.hpp `sct`).  Written as a chain of compares and selects it
+/// costs about two VALU instructions per constant; the path tracer's
+/// shading block runs eight of them over nine constants per bounce.  For a
+/// 4-byte element type and at least kSctTableMin constants the constants go
+/// into a table in LDS, one segment per instruction with a trailing 0, and
+/// the instruction is one clamp of the index (shared by every table lookup
+/// of the same index) and one LDS read at a constant offset.  Every kernel
+/// that runs blocks of @p K fills the table first (sct_table_init), with all
+/// its threads.
+inline constexpr std::size_t kSctTableMin = 4;
+
+template <typename K>
+struct SctTable
+{
+    static constexpr bool
+    uses(const ir::Instr& in)
+    {
+        return in.kind == SK::compute && in.op == ir::Op::sct && ir::ty_size(in.ety) == 4 &&
+               in.nc >= kSctTableMin;
+    }
+
+    struct Map
+    {
+        std::array<std::uint16_t, ir::table_of<K>.instr.size()> off{};
+        std::size_t                                            words = 0;
+    };
+
+    static constexpr Map map = []
+    {
+        Map m;
+        for(std::size_t i = 0; i < ir::table_of<K>.ni; ++i)
+        {
+            const ir::Instr& in = ir::table_of<K>.instr[i];
+            if(!uses(in)) continue;
+            m.off[i] = static_cast<std::uint16_t>(m.words);
+            m.words += in.nc + 1u;
+        }
+        return m;
+    }();
+    static constexpr std::size_t words = map.words;
+
+    /// the constants' bits in the element type (the low 32 bits of a 4-byte
+    /// constant, as const_as reads them), each segment ending in 0
+    static constexpr std::array<std::uint32_t, words ? words : 1> data = []
+    {
+        std::array<std::uint32_t, words ? words : 1> d{};
+        for(std::size_t i = 0; i < ir::table_of<K>.ni; ++i)
+        {
+            const ir::Instr& in = ir::table_of<K>.instr[i];
+            if(!uses(in)) continue;
+            for(std::size_t j = 0; j < in.nc; ++j)
+                d[map.off[i] + j] = static_cast<std::uint32_t>(ir::table_of<K>.konst[in.c0 + j]);
+            d[map.off[i] + in.nc] = 0;
+        }
+        return d;
+    }();
+};
+
+template <typename K>
+__shared__ std::uint32_t sct_lds[SctTable<K>::words ? SctTable<K>::words : 1];
+
+/// @brief Fill @p K's sct tables in LDS; every thread of the workgroup calls
+/// it before any of them runs a block of @p K (it ends in a barrier).
+template <typename K>
+LOOM_DEVICE void
+sct_table_init()
+{
+    if constexpr(SctTable<K>::words != 0)
+    {
+        if(threadIdx.x == 0)
+        {
+            [&]<std::size_t... W>(std::index_sequence<W...>)
+            {
+                ((sct_lds<K>[W] = SctTable<K>::data[W]), ...);
+            }(std::make_index_sequence<SctTable<K>::words>{});
+        }
+        __syncthreads();
+    }
+}
+
 /// @brief Run-time error flags raised by kernels.
 enum ErrorFlag : std::uint32_t
 {
@@ -278,6 +361,17 @@ exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
             const E y = conv<E>(operand<K, in.a0 + 1>(r));
             const E z = conv<E>(operand<K, in.a0 + 2>(r));
             r.template set<d.slot, D>(conv<D>(fma_<E>(x, y, op == Op::fma ? z : E(-z))));
+        }
+        else if constexpr(op == Op::sct && SctTable<K>::uses(in))
+        {
+            // index clamped to nc (the segment's trailing 0): negative and
+            // out-of-range indices select 0, as in the select chain below
+            using IX             = ctype<in.ity>;
+            using UX             = std::make_unsigned_t<IX>;
+            const UX          u  = static_cast<UX>(conv<IX>(operand<K, in.a0>(r)));
+            const std::uint32_t k = u < UX(in.nc) ? static_cast<std::uint32_t>(u) : in.nc;
+            const std::uint32_t bits = sct_lds<K>[SctTable<K>::map.off[I] + k];
+            r.template set<d.slot, D>(conv<D>(std::bit_cast<E>(bits)));
         }
         else if constexpr(op == Op::sct)
         {

@@ -759,6 +759,7 @@ template <typename K, int P>
 __global__ __launch_bounds__(256) void
 compact_kernel(IoArgs io, CtlArgs c)
 {
+    sct_table_init<K>();
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool          valid = tid < c.n;
     Regs<K>             r;
@@ -791,6 +792,7 @@ template <typename K, int P>
 __global__ __launch_bounds__(256) void
 compact_mem_kernel(IoArgs io, CtlArgs c, MemArgs a)
 {
+    sct_table_init<K>();
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool          valid = tid < c.n;
     MemRegs<K>          r{a.base, a.rows, static_cast<std::uint32_t>(tid)};
@@ -1079,6 +1081,7 @@ template <typename K, int P, bool RG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RG ? regroup_waves_hint<K> : waves_hint<K>))) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
+    sct_table_init<K>();
     HybridCtl* ctl            = h.ctl;
     const std::uint32_t par   = uniform(ctl->parity);
     const std::uint32_t nbq   = uniform(ctl->nbatches);

@@ -126,6 +126,7 @@ template <typename K, int P>
 __global__ __launch_bounds__(64) void
 stream_step_kernel(StreamArgs a, CtlArgs c)
 {
+    sct_table_init<K>();
     const std::uint64_t slot = std::uint64_t(blockIdx.x) * 64u + lane_id();
     const std::uint32_t pc0  = a.pc[slot];
     const bool          live = pc0 < kHaltPc;

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(64) void
 trace_kernel(IoArgs io, std::uint64_t n, std::uint32_t max_steps, std::uint64_t* rows,
              std::uint32_t* steps)
 {
+    sct_table_init<K>();
     const std::uint64_t tid = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(tid >= n) return;
     Regs<K> r;

@@ -204,6 +204,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
 {
     static_assert(W == 16 || W == 8, "a regrouping workgroup has 16 or 8 waves");
+    sct_table_init<K>();
     constexpr std::uint32_t T    = 64u * W;
     constexpr std::uint32_t NB   = ir::table_of<K>.nb;
     constexpr std::uint32_t KEYS = NB + 1; // blocks, then "no live thread"

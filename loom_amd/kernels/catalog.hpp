@@ -25,6 +25,8 @@
     X("roots_f64", loom_amd::programs::roots<double>)                 \
     X("literal_div_f32", loom_amd::programs::literal_div<float>)      \
     X("literal_div_f64", loom_amd::programs::literal_div<double>)     \
+    X("sct_range_f32", loom_amd::programs::sct_range<float>)          \
+    X("sct_range_f64", loom_amd::programs::sct_range<double>)         \
     X("threefry2x32_u32", loom_amd::programs::threefry2x32<std::uint32_t>) \
     X("cubic_bisect_f32", loom_amd::programs::cubic_bisect<float>)    \
     X("cubic_bisect_f64", loom_amd::programs::cubic_bisect<double>)

@@ -175,6 +175,20 @@ TKERNEL1(literal_div, x)
     TDIV(q6, 1, x)
 TKERNEL_END6(q1, q2, q3, q4, q5, q6);
 
+// Constant selection with indices inside and outside the constant list:
+// sct(i, c0, ..., cN-1) is c_i for i < N and 0 otherwise (operators.hpp
+// sct).  The backend reads a list of at least four 4-byte constants from a
+// table in LDS and the others through compares and selects; this kernel has
+// lists of both kinds.
+template <std::floating_point T>
+KERNEL(sct_range, IVAR(x))
+    TSCT6(a, x, 1.5, -2.25, 3e3, 0.125, -7, 9.5),
+    ISCT5(b, x, 7, 0xFFFF, 3, 0x12345678, 1),
+    ISUB(y, x, 2),
+    TSCT4(c, y, 0.5, 0.25, -1, 2),
+    TSCT3(s3, x, 4, 5, 6)
+KERNEL_END(TVAR(a), IVAR(b), TVAR(c), TVAR(s3));
+
 // Float op mix, straight-line, exercising the floating-point contract:
 // arithmetic, fused multiply-add/sub, sqrt/rsqrt, rounding, min/max,
 // copysign, sin/cos polynomials, select, union register and conversions.
