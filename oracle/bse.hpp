@@ -325,6 +325,10 @@ struct walk<loom::vm::Compute<O, D, As...>>
         s.kind = Stmt::compute;
         s.code = tr::code;
         s.dst  = make_arg<D>();
+        // a destination whose name parses as a number is a literal, which
+        // the reference rejects at compile time (core.hpp:966-970)
+        if(s.dst.literal)
+            throw std::runtime_error("destination named like a literal: " + s.dst.name);
         (s.args.push_back(make_arg<As>()), ...);
         if constexpr(!std::is_void_v<typename tr::type>)
             s.ety = type_code<typename tr::type>();
