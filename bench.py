@@ -110,7 +110,9 @@ def mixed_batch(steps):
 
 
 SERVE_CHECKED = 24  # requests per serving run whose outputs are compared
-WGR_PHASE = 128     # dispatches between two workgroup sorts (reference path tracer)
+# dispatches between two workgroup sorts on the reference path tracer, for
+# 16- and 8-wave workgroups (the fastest of 32-1024 for each size)
+WGR_PHASE = {16: 128, 8: 256}
 
 
 def serving(qps, count):
@@ -160,8 +162,8 @@ def ref_pt2_leg():
     of the compact sweep form, the hybrid form (refill, tuned live
     threshold), the hybrid form regrouping the threads of each workgroup by
     block through LDS every WGR_PHASE dispatches (16-wave workgroups, one per
-    CU, and 8-wave ones, two per CU), the compact form with its
-    register file in memory, and the queue-regrouping form under the
+    CU, and 8-wave ones, two per CU, each at its phase), the compact form
+    with its register file in memory, and the queue-regrouping form under the
     reference's successor-major policy at a block-population threshold of 32
     (regrouping passes: a wave stops when its best block has no more lanes
     than the threshold and its threads are queued by label for the next
@@ -181,9 +183,10 @@ def ref_pt2_leg():
         "compact_sweep": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP)),
         "hybrid_live56": (abi.MODE_HYBRID, dict(live_threshold=56)),
         "hybrid_workgroup_regroup": (abi.MODE_HYBRID, dict(
-            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE)),
+            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP,
+            regroup_phase=WGR_PHASE[16])),
         "hybrid_workgroup8_regroup": (abi.MODE_HYBRID, dict(
-            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE,
+            regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=WGR_PHASE[8],
             regroup_waves=8)),
         "compact_memory_registers": (abi.MODE_COMPACT, dict(reconv=abi.RECONV_SWEEP,
                                                             register_file=abi.REGFILE_MEMORY)),
