@@ -215,18 +215,20 @@ exact_reciprocal_bits()
     }
 }
 
-/// @brief Constant tables of `sct` (select a constant by index) in LDS.
+/// @brief Constant tables of `sct` (select a constant by index) and of
+/// multiway `ParmJump` in LDS.
 ///
 /// `sct(i, c0, ..., cN-1)` is c_i for 0 <= i < N and 0 otherwise
-/// (operators.hpp `sct`).  Written as a chain of compares and selects it
-/// costs about two VALU instructions per constant; the path tracer's
-/// shading block runs eight of them over nine constants per bounce.  For a
-/// 4-byte element type and at least kSctTableMin constants the constants go
-/// into a table in LDS, one segment per instruction with a trailing 0, and
-/// the instruction is one clamp of the index (shared by every table lookup
-/// of the same index) and one LDS read at a constant offset.  Every kernel
-/// that runs blocks of @p K fills the table first (sct_table_init), with all
-/// its threads.
+/// (operators.hpp `sct`); a multiway ParmJump on v goes to label v, or to
+/// HALT when v is past its label list.  Written as a chain of compares and
+/// selects either costs about two VALU instructions per entry; the path
+/// tracer's shading block runs eight selections of nine constants per
+/// bounce.  For a 4-byte element type (and every jump) with at least
+/// kSctTableMin entries the entries go into a table in LDS, one segment per
+/// instruction with a trailing 0 (HALT for a jump), and the instruction is
+/// one clamp of the index (shared by every table lookup of the same index)
+/// and one LDS read at a constant offset.  Every kernel that runs blocks of
+/// @p K fills the table first (sct_table_init), with all its threads.
 inline constexpr std::size_t kSctTableMin = 4;
 
 template <typename K>
@@ -235,8 +237,16 @@ struct SctTable
     static constexpr bool
     uses(const ir::Instr& in)
     {
+        if(in.kind == SK::pjump) return in.nt >= kSctTableMin;
         return in.kind == SK::compute && in.op == ir::Op::sct && ir::ty_size(in.ety) == 4 &&
                in.nc >= kSctTableMin;
+    }
+
+    /// entries of an instruction's segment before the trailing one
+    static constexpr std::size_t
+    entries(const ir::Instr& in)
+    {
+        return in.kind == SK::pjump ? in.nt : in.nc;
     }
 
     struct Map
@@ -253,14 +263,15 @@ struct SctTable
             const ir::Instr& in = ir::table_of<K>.instr[i];
             if(!uses(in)) continue;
             m.off[i] = static_cast<std::uint16_t>(m.words);
-            m.words += in.nc + 1u;
+            m.words += entries(in) + 1u;
         }
         return m;
     }();
     static constexpr std::size_t words = map.words;
 
     /// the constants' bits in the element type (the low 32 bits of a 4-byte
-    /// constant, as const_as reads them), each segment ending in 0
+    /// constant, as const_as reads them), each segment ending in 0; a jump's
+    /// target blocks, ending in HALT
     static constexpr std::array<std::uint32_t, words ? words : 1> data = []
     {
         std::array<std::uint32_t, words ? words : 1> d{};
@@ -268,6 +279,13 @@ struct SctTable
         {
             const ir::Instr& in = ir::table_of<K>.instr[i];
             if(!uses(in)) continue;
+            if(in.kind == SK::pjump)
+            {
+                for(std::size_t j = 0; j < in.nt; ++j)
+                    d[map.off[i] + j] = ir::table_of<K>.target[in.t0 + j];
+                d[map.off[i] + in.nt] = HALT;
+                continue;
+            }
             for(std::size_t j = 0; j < in.nc; ++j)
                 d[map.off[i] + j] = static_cast<std::uint32_t>(ir::table_of<K>.konst[in.c0 + j]);
             d[map.off[i] + in.nc] = 0;
@@ -321,6 +339,14 @@ exec_instr(Regs<K>& r, std::uint32_t& pc, std::uint32_t& err)
         if constexpr(in.nt == 2)
             pc = v != decltype(v)(0) ? LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0 + 1])) :
                                        LOOM_CX(std::uint32_t(ir::table_of<K>.target[in.t0]));
+        else if constexpr(SctTable<K>::uses(in))
+        {
+            // index clamped to nt (the segment's trailing HALT)
+            using UX = std::make_unsigned_t<decltype(v)>;
+            const UX            u = static_cast<UX>(v);
+            const std::uint32_t k = u < UX(in.nt) ? static_cast<std::uint32_t>(u) : in.nt;
+            pc                    = sct_lds<K>[SctTable<K>::map.off[I] + k];
+        }
         else
         {
             std::uint32_t p = HALT;
