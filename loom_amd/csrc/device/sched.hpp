@@ -188,9 +188,10 @@ inline constexpr std::uint32_t kFlushSlots = 1u << 30;
 /// one sweep late.
 inline constexpr std::uint32_t kSegment = 128;
 
-/// Blocks of at least this many ops count their thread steps with a scalar
-/// popcount of the block mask; smaller blocks (whose waves are short of
-/// scalar issue, not of vector registers) with a per-lane add.
+/// Blocks of at least this many ops count their work (thread steps, active
+/// operations and flops) from a scalar popcount of the block mask, on the
+/// scalar unit; smaller blocks (whose waves are short of scalar issue, not of
+/// vector issue or registers) with per-lane adds.
 inline constexpr std::uint32_t kScalarStepOps = 8;
 
 /// @brief Per-wave scheduling state.  Useful work is counted per lane (a
@@ -208,6 +209,8 @@ struct WaveCtx
     std::uint32_t  left       = 0; ///< wave-uniform: dispatches of the current segment
     std::uint32_t  slots      = 0; ///< wave-uniform: Σ 64·ops(B) since the last flush
     std::uint32_t  steps      = 0; ///< wave-uniform: Σ popcount(EXEC) of large blocks since the last flush
+    std::uint32_t  ops        = 0; ///< wave-uniform: Σ popcount(EXEC)·ops(B) of large blocks since the last flush
+    std::uint32_t  flops      = 0; ///< wave-uniform: Σ popcount(EXEC)·flops(B) of large blocks since the last flush
     std::uint32_t  lane_steps = 0; ///< this lane: runs of small blocks
     std::uint32_t  lane_ops   = 0; ///< this lane: Σ ops(B) of the blocks it ran
     std::uint32_t  lane_flops = 0; ///< this lane: Σ flops(B) of the blocks it ran
@@ -258,9 +261,10 @@ flush_stats(WaveCtx& w)
     if(w.stats)
     {
         // a lane's ops are below slots/64 < 2^25 and its flops below twice
-        // that, so the wave totals fit 32 bits
-        const std::uint64_t act   = wave_sum(w.lane_ops);
-        const std::uint64_t flops = wave_sum(w.lane_flops);
+        // that, so the wave totals (and the scalar sums of large blocks,
+        // below slots and twice slots) fit 32 bits
+        const std::uint64_t act   = std::uint64_t(w.ops) + wave_sum(w.lane_ops);
+        const std::uint64_t flops = std::uint64_t(w.flops) + wave_sum(w.lane_flops);
         const std::uint64_t steps = std::uint64_t(w.steps) + wave_sum(w.lane_steps);
         const std::uint64_t total = w.issued + w.disp;
         if(lane_id() == 0)
@@ -276,6 +280,8 @@ flush_stats(WaveCtx& w)
     }
     w.slots      = 0;
     w.steps      = 0;
+    w.ops        = 0;
+    w.flops      = 0;
     w.lane_steps = 0;
     w.lane_ops   = 0;
     w.lane_flops = 0;
@@ -333,11 +339,18 @@ run_block(RF& r, std::uint32_t& pc, WaveCtx& w)
 {
     constexpr ir::Block blk = ir::table_of<K>.block[B];
     static_assert(blk.flops <= 2u * blk.ops, "flops exceed twice the op count");
-    if constexpr(blk.ops >= kScalarStepOps) w.steps += popc(ballot(pc == B));
-    if(pc == B)
+    if constexpr(blk.ops >= kScalarStepOps)
+    {
+        const std::uint32_t n = popc(ballot(pc == B));
+        w.steps += n;
+        w.ops += n * blk.ops;
+        if constexpr(blk.flops != 0) w.flops += n * blk.flops;
+        if(pc == B) exec_on<K, B>(r, pc, w.err);
+    }
+    else if(pc == B)
     {
         exec_on<K, B>(r, pc, w.err);
-        if constexpr(blk.ops < kScalarStepOps) w.lane_steps += 1;
+        w.lane_steps += 1;
         w.lane_ops += blk.ops;
         if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
     }
