@@ -316,8 +316,9 @@ def cpu_model() -> str:
 def integer_configs(steps: int):
     """fib u64 over 2^20 threads and Collatz u64 over 2^24 threads, inputs in
     HBM: the form the autotuner chose, the tuned hybrid form and the compact
-    form under the reference policy and the sweep policy, threads/s and
-    active lanes, outputs compared across forms."""
+    form under the reference policy and the sweep policy, threads/s (timed
+    without statistics) and active lanes (from a run with them), outputs
+    compared across forms."""
     import loom_amd
     from loom_amd import abi, autotune, workloads
     out = {}
@@ -340,11 +341,12 @@ def integer_configs(steps: int):
         entry, first = {"threads": n, "tuned_form": {"mode": tmode, "params": tparams}}, None
         for form, (mode, params) in forms.items():
             dout = [loom_amd.DeviceArray(n, dt) for dt in k.outputs]
-            loom_amd.run(k, mode, din, dout, params=params)
-            ms, r = [], None
+            loom_amd.run(k, mode, din, dout, params=params)  # with statistics
+            r = loom_amd.run(k, mode, din, dout, params=params)
+            params.collect_stats = 0  # timed without
+            ms = []
             for _ in range(max(steps, 1)):
-                r = loom_amd.run(k, mode, din, dout, params=params)
-                ms.append(r.ms_total)
+                ms.append(loom_amd.run(k, mode, din, dout, params=params).ms_total)
             got = [d.numpy() for d in dout]
             if first is None:
                 first = got
@@ -380,9 +382,13 @@ def main():
     ins = [loom_amd.DeviceArray.from_numpy(x) for x in workloads.pt_inputs(W, H, SPP)]
     outs = [loom_amd.DeviceArray(n, np.float32) for _ in range(3)]
 
+    # timed steps collect no statistics (the production setting); the work
+    # counters (active lanes, flops) come from one more run of each form
+    # with statistics, whose outputs are the same
     hyb = autotune.best_params("smallpt9_f32")
-    hp = loom_amd.make_params(mode=abi.MODE_HYBRID, **hyb)
-    cp = loom_amd.make_params(mode=abi.MODE_COMPACT)
+    hp = loom_amd.make_params(mode=abi.MODE_HYBRID, stats=False, **hyb)
+    hp_stats = loom_amd.make_params(mode=abi.MODE_HYBRID, **hyb)
+    cp = loom_amd.make_params(mode=abi.MODE_COMPACT, stats=False)
 
     def steps(mode, params, count):
         last = None
@@ -395,6 +401,7 @@ def main():
     steps(abi.MODE_HYBRID, hp, a.warmup)
     h_ms, h_last = steps(abi.MODE_HYBRID, hp, a.steps)
     h_img = [o.numpy() for o in outs]
+    h_stats = loom_amd.run(k, abi.MODE_HYBRID, ins, outs, params=hp_stats)
 
     result = {
         "metric": METRIC,
@@ -413,7 +420,7 @@ def main():
                                f"sample ({n} threads), 9-sphere scene, max depth 32",
                    "width": W, "height": H, "spp": SPP, "samples": n,
                    "backend": "hybrid", "hybrid_params": hyb},
-        "active_lane_fraction": h_last.active_lane_fraction,
+        "active_lane_fraction": h_stats.active_lane_fraction,
     }
     # the measured frame against the committed whole-frame fixture (SHA-256
     # of each radiance channel, from the direct CPU path tracer)
@@ -430,12 +437,14 @@ def main():
         same = True
         for key, params, label in (
                 ("compact", cp, "successor-major (reference policy)"),
-                ("compact_sweep", loom_amd.make_params(mode=abi.MODE_COMPACT,
+                ("compact_sweep", loom_amd.make_params(mode=abi.MODE_COMPACT, stats=False,
                                                        reconv=abi.RECONV_SWEEP),
                  "sweep (every block in topological order)")):
             steps(abi.MODE_COMPACT, params, 1)
-            c_ms, c_last = steps(abi.MODE_COMPACT, params, a.steps)
+            c_ms, _ = steps(abi.MODE_COMPACT, params, a.steps)
             c_img = [o.numpy() for o in outs]
+            params.collect_stats = 1
+            c_last = loom_amd.run(k, abi.MODE_COMPACT, ins, outs, params=params)
             same = same and all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
                                 for x, y in zip(h_img, c_img))
             result[key] = {"value": n / (c_ms / 1e3), "ms_per_step": c_ms,
@@ -447,7 +456,7 @@ def main():
     # roofline of the dominant kernel: the hybrid pass kernel
     launches = max(h_last.main_launches, 1)
     launch_ms = h_last.ms_main / launches
-    flops = h_last.counters[abi.C_ACTIVE_FLOPS] / launches
+    flops = h_stats.counters[abi.C_ACTIVE_FLOPS] / max(h_stats.main_launches, 1)
     achieved = flops / (launch_ms / 1e3) / 1e12
     result["roofline"] = {
         "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

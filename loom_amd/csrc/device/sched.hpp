@@ -332,6 +332,34 @@ segment_check(WaveCtx& w)
 inline constexpr std::uint32_t kMayHalt = 1u << 16;
 inline constexpr std::uint32_t kIndex   = 0xFFFFu;
 
+/// @brief The register file of a run that collects no statistics: small
+/// blocks then skip their per-lane work counts, which cost a kernel made of
+/// short blocks (Collatz's loop) about an eighth of its time.  Large blocks
+/// count on the scalar unit either way.
+template <typename K>
+struct Uncounted : Regs<K>
+{
+};
+
+template <typename RF>
+inline constexpr bool counts_small_blocks = true;
+template <typename K>
+inline constexpr bool counts_small_blocks<Uncounted<K>> = false;
+
+/// @brief Kernel @p K has a block below kScalarStepOps ops (only such
+/// kernels are compiled a second time without per-lane counts).
+template <typename K>
+inline constexpr bool has_small_blocks = []
+{
+    for(std::size_t b = 0; b < ir::table_of<K>.nb; ++b)
+        if(ir::table_of<K>.block[b].ops < kScalarStepOps) return true;
+    return false;
+}();
+
+/// @brief The register file of kernel @p K's runs: counted, or not (CNT).
+template <typename K, bool CNT>
+using RegsOf = std::conditional_t<CNT, Regs<K>, Uncounted<K>>;
+
 /// @brief Run block @p B on the lanes at B and count the work.
 template <typename K, std::size_t B, typename RF>
 LOOM_DEVICE void
@@ -350,9 +378,12 @@ run_block(RF& r, std::uint32_t& pc, WaveCtx& w)
     else if(pc == B)
     {
         exec_on<K, B>(r, pc, w.err);
-        w.lane_steps += 1;
-        w.lane_ops += blk.ops;
-        if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
+        if constexpr(counts_small_blocks<RF>)
+        {
+            w.lane_steps += 1;
+            w.lane_ops += blk.ops;
+            if constexpr(blk.flops != 0) w.lane_flops += blk.flops;
+        }
     }
     w.slots += 64u * blk.ops;
     w.disp += 1;
@@ -768,14 +799,14 @@ raise_errors(std::uint32_t lane_err, bool over, const CtlArgs& c)
 // ---------------------------------------------------------------------------
 /// @brief Predicated wave64 kernel: each wave runs 64 consecutive threads to
 /// completion (reference compact_compiler, simd.hpp:3742-4014).
-template <typename K, int P>
+template <typename K, int P, bool CNT = true>
 __global__ __launch_bounds__(256) void
 compact_kernel(IoArgs io, CtlArgs c)
 {
     sct_table_init<K>();
     const std::uint64_t tid   = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool          valid = tid < c.n;
-    Regs<K>             r;
+    RegsOf<K, CNT>      r;
     std::uint32_t       pc = HALT;
     if(valid)
     {
@@ -1090,7 +1121,7 @@ take_input(HybridCtl* ctl, std::uint32_t home, Reservoir& res, std::uint32_t wan
 /// kernel needs far fewer VGPRs for a large register file (the reference's
 /// pt2_const: 239 -> 138, two -> three waves per SIMD), so runs that do not
 /// regroup launch the RG = false kernel, in which no queue is ever read.
-template <typename K, int P, bool RG>
+template <typename K, int P, bool RG, bool CNT = true>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RG ? regroup_waves_hint<K> : waves_hint<K>))) void
 hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
 {
@@ -1124,10 +1155,10 @@ hybrid_pass_kernel(IoArgs io, CtlArgs c, HybridArgs h)
                   admitted = 0, refills = 0, batches = 0, spill_failed = 0;
     Reservoir res;
 
-    Regs<K>       r;
-    std::uint32_t pc  = HALT;
-    std::uint32_t tid = 0;
-    bool          has = false;
+    RegsOf<K, CNT> r;
+    std::uint32_t  pc  = HALT;
+    std::uint32_t  tid = 0;
+    bool           has = false;
 
     for(;;)
     {

@@ -199,7 +199,7 @@ wgr_round(Regs<K>& r, std::uint32_t old_key, std::uint32_t new_key, std::uint32_
 /// one another.  Both sizes are compiled for four waves per SIMD (128
 /// VGPRs), which the 16-wave workgroup needs to be resident at all and
 /// which lets two 8-wave workgroups share a CU.
-template <typename K, int P, int W>
+template <typename K, int P, int W, bool CNT = true>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void
 wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
 {
@@ -216,7 +216,7 @@ wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
     __shared__ std::uint32_t buf[kWgrRoundWords * T];
 
     const std::uint32_t me  = threadIdx.x;
-    Regs<K>             r;
+    RegsOf<K, CNT>      r;
     std::uint32_t       pc  = HALT;
     std::uint32_t       tid = kNoTid;
     WaveCtx             w(c);

@@ -122,6 +122,15 @@ launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
 {
     if(c.n == 0) return hipSuccess;
     const std::uint64_t blocks = (c.n + 255) / 256;
+    // a run without statistics takes the kernel without per-lane work
+    // counts where the kernel has small blocks (the same kernel otherwise)
+    if constexpr(has_small_blocks<K>)
+        if(!c.stats)
+        {
+            hipLaunchKernelGGL((compact_kernel<K, P, false>), dim3(static_cast<unsigned>(blocks)),
+                               dim3(256), 0, s, io, c);
+            return hipGetLastError();
+        }
     hipLaunchKernelGGL((compact_kernel<K, P>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                        io, c);
     return hipGetLastError();
@@ -144,6 +153,17 @@ launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, int waves, unsi
            hipStream_t s)
 {
     if(c.n == 0) return hipSuccess;
+    if constexpr(has_small_blocks<K>)
+        if(!c.stats)
+        {
+            if(waves == 8)
+                hipLaunchKernelGGL((wgr_kernel<K, P, 8, false>), dim3(grid), dim3(64 * 8), 0, s, io,
+                                   c, g);
+            else
+                hipLaunchKernelGGL((wgr_kernel<K, P, 16, false>), dim3(grid), dim3(64 * 16), 0, s,
+                                   io, c, g);
+            return hipGetLastError();
+        }
     if(waves == 8)
         hipLaunchKernelGGL((wgr_kernel<K, P, 8>), dim3(grid), dim3(64 * 8), 0, s, io, c, g);
     else
@@ -166,7 +186,9 @@ wgr_occupancy(int waves)
 }
 
 /// Runs that regroup launch the pass kernel with the queue paths compiled
-/// in; the others the kernel without them.
+/// in; the others the kernel without them, and those of them that collect
+/// no statistics, for a kernel with small blocks, the one without per-lane
+/// work counts.
 template <typename K, int P>
 hipError_t
 launch_hybrid_pass(const IoArgs& io, const CtlArgs& c, const HybridArgs& h, unsigned grid,
@@ -174,6 +196,9 @@ launch_hybrid_pass(const IoArgs& io, const CtlArgs& c, const HybridArgs& h, unsi
 {
     if(c.regroup)
         hipLaunchKernelGGL((hybrid_pass_kernel<K, P, true>), dim3(grid), dim3(64), 0, s, io, c, h);
+    else if(has_small_blocks<K> && !c.stats)
+        hipLaunchKernelGGL((hybrid_pass_kernel<K, P, false, !has_small_blocks<K>>), dim3(grid),
+                           dim3(64), 0, s, io, c, h);
     else
         hipLaunchKernelGGL((hybrid_pass_kernel<K, P, false>), dim3(grid), dim3(64), 0, s, io, c, h);
     return hipGetLastError();
