@@ -16,8 +16,8 @@
 ///             (one atomic per distinct key of a wave) and its prefix give
 ///             each thread its position in block order;
 ///   exchange  pc and thread id, then the registers live at the thread's
-///             block, move through an LDS buffer, a fixed range of
-///             kWgrRoundWords register words per round: each lane writes its
+///             block, move through an LDS buffer, a fixed range of at
+///             most kWgrRoundWords register words per round: each lane writes its
 ///             thread's live words at the thread's position and reads the
 ///             live words of the thread at its own index;
 ///   retire    the lanes at HALT (the last ones after the sort) store the
@@ -38,8 +38,12 @@ namespace loom::hipvm
 {
 /// @brief Thread id of a lane that holds no thread.
 inline constexpr std::uint32_t kNoTid = 0xFFFFFFFFu;
-/// @brief State words exchanged per thread per LDS round.
-inline constexpr std::size_t kWgrRoundWords = 15;
+/// @brief Most state words exchanged per thread per LDS round.  A kernel
+/// moves its n words in ceil(n / kWgrRoundWords) rounds of equal size (the
+/// LDS buffer holds one round), so a small register file keeps a small
+/// buffer and the occupancy it allows.
+inline constexpr std::size_t kWgrRoundWords = 31;
+
 /// @brief Default waves of a regrouping workgroup: 16 (one workgroup holds
 /// every lane of a CU at four waves per SIMD); the other size is 8 (two
 /// workgroups per CU, so one runs while the other sorts).
@@ -108,6 +112,18 @@ struct WgrWords
     }();
 };
 
+/// @brief Rounds of kernel @p K's exchange and words per round.
+template <typename K>
+struct WgrRounds
+{
+    static constexpr std::size_t n      = WgrWords<K>::n;
+    static constexpr std::size_t rounds = n ? (n + kWgrRoundWords - 1) / kWgrRoundWords : 0;
+    static constexpr std::size_t even   = rounds ? (n + rounds - 1) / rounds : 0;
+    /// at least two: the pc and thread-id exchange uses the buffer's first
+    /// two rows
+    static constexpr std::size_t words = even < 2 ? 2 : even;
+};
+
 template <typename K, std::size_t Wd>
 LOOM_DEVICE std::uint32_t
 wgr_word(const Regs<K>& r)
@@ -150,12 +166,12 @@ wgr_on_key(std::uint32_t key, F&& f)
     }(std::make_index_sequence<WgrWords<K>::NB + 1>{});
 }
 
-/// @brief One exchange round of words [W0, W0 + kWgrRoundWords): a thread's
+/// @brief One exchange round of words [W0, W0 + RW): a thread's
 /// live words go to its position @p pos (key @p old_key), and this lane
 /// takes the live words of the thread now at its index @p me (key
 /// @p new_key, known from the pc word exchanged first).  Both barriers are
 /// reached by every lane.
-template <typename K, std::size_t W0, std::uint32_t T>
+template <typename K, std::size_t W0, std::size_t RW, std::uint32_t T>
 LOOM_DEVICE void
 wgr_round(Regs<K>& r, std::uint32_t old_key, std::uint32_t new_key, std::uint32_t* buf,
           std::uint32_t pos, std::uint32_t me)
@@ -173,7 +189,7 @@ wgr_round(Regs<K>& r, std::uint32_t old_key, std::uint32_t new_key, std::uint32_
                         buf[J * T + pos] = wgr_word<K, W0 + J>(r);
                 }(),
                 ...);
-        }(std::make_index_sequence<kWgrRoundWords>{});
+        }(std::make_index_sequence<RW>{});
     });
     __syncthreads();
     wgr_on_key<K>(new_key, [&](auto kc)
@@ -188,7 +204,7 @@ wgr_round(Regs<K>& r, std::uint32_t old_key, std::uint32_t new_key, std::uint32_
                         wgr_set_word<K, W0 + J>(r, buf[J * T + me]);
                 }(),
                 ...);
-        }(std::make_index_sequence<kWgrRoundWords>{});
+        }(std::make_index_sequence<RW>{});
     });
     __syncthreads();
 }
@@ -208,12 +224,13 @@ wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
     constexpr std::uint32_t T    = 64u * W;
     constexpr std::uint32_t NB   = ir::table_of<K>.nb;
     constexpr std::uint32_t KEYS = NB + 1; // blocks, then "no live thread"
-    constexpr std::size_t   NR   = (WgrWords<K>::n + kWgrRoundWords - 1) / kWgrRoundWords;
+    constexpr std::size_t   NR   = WgrRounds<K>::rounds;
+    constexpr std::size_t   RW   = WgrRounds<K>::words;
     static_assert(KEYS <= T, "more blocks than lanes");
     __shared__ std::uint32_t hist[KEYS];
     __shared__ std::uint32_t cur[KEYS];
     __shared__ std::uint32_t s_empty, s_got, s_base_lo, s_base_hi, s_stop;
-    __shared__ std::uint32_t buf[kWgrRoundWords * T];
+    __shared__ std::uint32_t buf[RW * T];
 
     const std::uint32_t me  = threadIdx.x;
     RegsOf<K, CNT>      r;
@@ -279,7 +296,7 @@ wgr_kernel(IoArgs io, CtlArgs c, WgrArgs g)
         __syncthreads();
         const std::uint32_t nkey = pc < NB ? pc : NB;
         [&]<std::size_t... Rd>(std::index_sequence<Rd...>)
-        { (wgr_round<K, Rd * kWgrRoundWords, T>(r, key, nkey, buf, pos, me), ...); }(std::make_index_sequence<NR>{});
+        { (wgr_round<K, Rd * RW, RW, T>(r, key, nkey, buf, pos, me), ...); }(std::make_index_sequence<NR>{});
 
         // ---- retire finished threads, claim new ones for the empty lanes ----
         if(pc == HALT && tid != kNoTid)
