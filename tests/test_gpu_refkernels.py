@@ -78,7 +78,10 @@ def test_reference_kernel_gpu_sized_frame(gpu):
                                                        register_file=abi.REGFILE_MEMORY)),
              ("hybrid_sweep", abi.MODE_HYBRID, {}),
              ("hybrid_workgroup_regroup", abi.MODE_HYBRID,
-              dict(regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=128))]
+              dict(regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=128)),
+             ("hybrid_workgroup8_regroup", abi.MODE_HYBRID,
+              dict(regroup=abi.REGROUP_WORKGROUP, reconv=abi.RECONV_SWEEP, regroup_phase=256,
+                   regroup_waves=8))]
     for fname, mode, params in forms:
         got = loom_amd.run(k, mode, [pick], params=loom_amd.make_params(mode=mode, **params))
         assert got.errors == 0
