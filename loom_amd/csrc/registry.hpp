@@ -125,12 +125,14 @@ launch_compact(const IoArgs& io, const CtlArgs& c, hipStream_t s)
     // a run without statistics takes the kernel without per-lane work
     // counts where the kernel has small blocks (the same kernel otherwise)
     if constexpr(has_small_blocks<K>)
+    {
         if(!c.stats)
         {
             hipLaunchKernelGGL((compact_kernel<K, P, false>), dim3(static_cast<unsigned>(blocks)),
                                dim3(256), 0, s, io, c);
             return hipGetLastError();
         }
+    }
     hipLaunchKernelGGL((compact_kernel<K, P>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                        io, c);
     return hipGetLastError();
@@ -154,6 +156,7 @@ launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, int waves, unsi
 {
     if(c.n == 0) return hipSuccess;
     if constexpr(has_small_blocks<K>)
+    {
         if(!c.stats)
         {
             if(waves == 8)
@@ -164,6 +167,7 @@ launch_wgr(const IoArgs& io, const CtlArgs& c, const WgrArgs& g, int waves, unsi
                                    io, c, g);
             return hipGetLastError();
         }
+    }
     if(waves == 8)
         hipLaunchKernelGGL((wgr_kernel<K, P, 8>), dim3(grid), dim3(64 * 8), 0, s, io, c, g);
     else
