@@ -112,9 +112,9 @@ def tune(kernel_name: str, inputs, reps: int = 2, space: dict | None = None,
     for mode, cand in cands:
         m, p = params_of_form(mode, {**cand, "stats": True})
         act = loom_amd.run(k, m, din, dout, params=p).active_lane_fraction  # warm
-        # timed without counters, the form as it is used; short runs are
+        # timed without statistics, the form as it is used; short runs are
         # repeated until about 20 ms have passed
-        m, p = params_of_form(mode, cand)
+        m, p = params_of_form(mode, {**cand, "stats": False})
         t0 = time.perf_counter()
         loom_amd.run(k, m, din, dout, params=p)
         first = (time.perf_counter() - t0) * 1e3
